@@ -1,0 +1,211 @@
+#!/usr/bin/env python3
+"""Benchmark: batched ML-KEM NTT + inverse NTT round trip on one MI355X.
+
+A step is one forward NTT pass and one inverse NTT pass over the whole batch
+(2^20 polys of the edge-value mix, seed 3: 70% uniform in [-3328, 3328], 10%
+zero, 10% all 3328, 5% all -3328, 5% unit impulse), inputs resident in HBM:
+forward out-of-place (in -> buf), inverse in place (buf).  The throughput is
+npoly / (t_fwd + t_inv), the round-trip rate the reference reports
+(ntt_bench.cu:480-488), for the INT8-MFMA method; the plain butterfly kernel
+is timed on the same inputs in the same run, and the reference C code on the
+host cores (``cpu_baseline``).
+
+Prints one JSON line.  ``--gpus N`` with N > 1 exits: this build measures one
+GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_POLY_PASS = 1024  # 512 B read + 512 B written per poly per transform
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--npoly", type=int, default=1 << 20)
+    ap.add_argument("--method", default="mfma16")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-check", action="store_true", help="skip the parity check")
+    ap.add_argument("--only", default=None, help="time only this method (profiling runs)")
+    return ap.parse_args()
+
+
+def time_method(tk, torch, method, src, buf, steps, warmup, npoly):
+    """Times `steps` round trips; returns per-kernel mean ms and wall ms/step."""
+    stream = torch.cuda.current_stream()
+    for _ in range(warmup):
+        tk.ntt(src, out=buf, method=method)
+        tk.invntt(buf, method=method)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3 * steps)]
+    t0 = time.perf_counter()
+    for s in range(steps):
+        ev[3 * s].record(stream)
+        tk.ntt(src, out=buf, method=method)
+        ev[3 * s + 1].record(stream)
+        tk.invntt(buf, method=method)
+        ev[3 * s + 2].record(stream)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) * 1e3 / steps
+    f = np.array([ev[3 * s].elapsed_time(ev[3 * s + 1]) for s in range(steps)])
+    i = np.array([ev[3 * s + 1].elapsed_time(ev[3 * s + 2]) for s in range(steps)])
+    rt = npoly / ((f + i) / 1e3)
+    return {
+        "fwd_ms": float(f.mean()), "inv_ms": float(i.mean()),
+        "fwd_ms_median": float(np.median(f)), "inv_ms_median": float(np.median(i)),
+        "rt_poly_s_mean": float(rt.mean()), "rt_poly_s_median": float(np.median(rt)),
+        "rt_poly_s_sd": float(rt.std(ddof=1)) if steps > 1 else 0.0,
+        "wall_ms_per_step": wall,
+    }
+
+
+def cpu_baseline(x):
+    """Reference C ntt()+invntt() round trip on the host cores (bounded sample)."""
+    from oracle.oracle import Oracle, Reference, reference_available
+    kind = "reference" if reference_available() else "port"
+    impl = Reference() if kind == "reference" else Oracle()
+    cores = impl.max_threads()
+    npoly = min(x.shape[0], 1 << 18)
+    sample = np.ascontiguousarray(x[:npoly])
+    reps, t, polys = 0, 0.0, 0
+    while t < 3.0 and reps < 20:
+        a = sample.copy()
+        t0 = time.perf_counter()
+        impl.roundtrip_inplace(a)
+        t += time.perf_counter() - t0
+        polys += npoly
+        reps += 1
+    return {"value": polys / t, "unit": "poly/s", "cores": cores, "kind": kind,
+            "sample": f"{reps} x {npoly} polys of the same workload, ntt() then invntt() "
+                      f"per poly, OpenMP over polys on {cores} host threads, "
+                      f"{t:.2f} s wall ({t * cores:.1f} core-s)"}
+
+
+def check_parity(tk, torch, method, x, buf_fwd, buf_rt):
+    """Canonical equality of the GPU forward and inverse against the oracle."""
+    from oracle.oracle import Oracle
+    o = Oracle()
+    fwd = buf_fwd.cpu().numpy()
+    ok_f = np.array_equal(fwd, np.mod(o.ntt(x).astype(np.int32), 3329).astype(np.int16))
+    inv = buf_rt.cpu().numpy()
+    want = np.mod(o.invntt(fwd).astype(np.int32), 3329)
+    want = np.where(want > 1664, want - 3329, want).astype(np.int16)
+    return bool(ok_f and np.array_equal(inv, want))
+
+
+def load_traffic(kernel):
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f).get(kernel)
+    except (OSError, ValueError):
+        return None
+
+
+def main():
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the batch shards by independent polys with no "
+              "exchange, so an N-GPU figure is N single-GPU runs; run with --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import tensorkyber_amd as tk
+    tk.load_library()
+    assert torch.cuda.is_available(), "no GPU visible"
+    dev = torch.device("cuda:0")
+    rc = tk.lib().tk_init()
+    if rc != 0:
+        raise SystemExit(f"tk_init failed: {rc}")
+
+    npoly = args.npoly
+    x = tk.workload(3, npoly)
+    src = torch.from_numpy(x).to(dev)
+    buf = torch.empty_like(src)
+    torch.cuda.synchronize()
+
+    methods = [args.only] if args.only else [args.method, "butterfly"]
+    res = {}
+    for m in methods:
+        print(f"[bench] timing {m} npoly={npoly} steps={args.steps}", file=sys.stderr, flush=True)
+        res[m] = time_method(tk, torch, m, src, buf, args.steps, args.warmup, npoly)
+
+    main_m = methods[0]
+    r = res[main_m]
+    value = npoly / ((r["fwd_ms"] + r["inv_ms"]) / 1e3)
+    dom = "fwd" if r["fwd_ms"] >= r["inv_ms"] else "inv"
+    dom_ms = r[f"{dom}_ms"]
+    achieved = BYTES_PER_POLY_PASS * npoly / (dom_ms / 1e3) / 1e9
+    kname = f"{main_m}_{dom}"
+
+    bit_exact = None
+    if not args.no_check:
+        print("[bench] parity check", file=sys.stderr, flush=True)
+        fwd = torch.empty_like(src)
+        tk.ntt(src, out=fwd, method=main_m)
+        rt = fwd.clone()
+        tk.invntt(rt, method=main_m)
+        torch.cuda.synchronize()
+        bit_exact = check_parity(tk, torch, main_m, x, fwd, rt)
+
+    cpu = None
+    if not args.no_cpu:
+        print("[bench] cpu baseline", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(x)
+
+    line = {
+        "metric": "NTT+iNTT poly/s at 2^20 polys (MFMA vs butterfly, bit-exact); % HBM roofline",
+        "value": value,
+        "unit": "poly/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["fwd_ms"] + r["inv_ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "i8xi8->i32 (int16 coefficients)",
+        "data": "synthetic",
+        "config": {
+            "workload": f"edge-value mix, {npoly} polys x 256 int16 (seed 3), "
+                        "ntt (out of place) + invntt (in place), inputs resident in HBM",
+            "npoly": npoly,
+            "method": main_m,
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": kname,
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": load_traffic(kname),
+        },
+        "rt_fraction_of_roofline": value * 2 * BYTES_PER_POLY_PASS / 1e9 / HBM_PEAK_GBS,
+        "bit_exact": bit_exact,
+        "kernels": res,
+        "cpu_baseline": cpu,
+    }
+    if "butterfly" in res and main_m != "butterfly":
+        b = res["butterfly"]
+        line["butterfly_poly_s"] = npoly / ((b["fwd_ms"] + b["inv_ms"]) / 1e3)
+        line["speedup_vs_butterfly"] = value / line["butterfly_poly_s"]
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,170 @@
+/*
+ * tkyber.h -- C ABI of the MI355X batched ML-KEM NTT engine.
+ *
+ * Ring R_q = Z_3329[x]/(x^256 + 1).  A polynomial is int16_t[256] in the
+ * reference's coefficient order; a batch is poly-major [npoly][256] in device
+ * memory (HBM), 16-byte aligned.  Every entry point takes an explicit stream
+ * (a hipStream_t passed as void*, NULL = the legacy default stream), never
+ * allocates on the hot path, never exits the process, and returns 0 or a
+ * negative tk_status.  All arguments are validated before anything is
+ * launched.  Calls from any host thread are allowed; calls on one stream
+ * execute in order.
+ *
+ * Drop-in mapping to the reference (ntt_bench/ in benlwk/TensorKyber):
+ *
+ *   ntt_batch      replaces ntt(int16_t r[256])              ref/ntt.h:10-11, ref/ntt.c:80-95
+ *                  and the batch kernels k_ntt_2tc_tc_cc /   ntt_bench.cu:61-62,181-182
+ *                  k_ntt_tc_cc_cp (int16_t* d, int npoly)
+ *   invntt_batch   replaces invntt(int16_t r[256])           ref/ntt.h:13-14, ref/ntt.c:106-126
+ *                  and k_invntt_2tc_tc_cc / k_invntt_tc_cc_cp ntt_bench.cu:97-98,230-231
+ *   basemul_batch  replaces basemul(r, a, b, zeta) applied   ref/ntt.h:16-17, ref/ntt.c:139-146
+ *                  to the 128 degree-1 pairs of each poly
+ *
+ * Output contract (the reference checks mod-q equality after canonicalising
+ * both sides, ntt_bench.cu:397-421):
+ *   - ntt_batch writes canonical coefficients in [0, q) (the form of the
+ *     reference's WMMA forward, ntt_kernels.cuh:448-455);
+ *   - invntt_batch writes centred coefficients in [-(q-1)/2, (q-1)/2] (the
+ *     form of the reference's WMMA inverse, ntt_kernels.cuh:614-621);
+ *   - the _ex variants select the form, including TK_FORM_REF, which yields
+ *     the int16 values of ntt()/invntt() themselves (butterfly method only).
+ * Inputs may be any int16 values.  The MFMA method computes the exact
+ * transform mod q for every input; ref ntt() itself is exact mod q whenever
+ * its int16 intermediates do not wrap, which holds for |x| <= 16927 (ntt) and
+ * |x| <= 16383 (invntt).
+ */
+#ifndef TKYBER_H
+#define TKYBER_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TK_N 256
+#define TK_Q 3329
+
+typedef enum {
+  TK_OK = 0,
+  TK_EINVAL = -22,   /* null / misaligned pointer, zero size, bad enum, bad table */
+  TK_ENOMEM = -12,   /* device allocation failed (init / plan creation only)      */
+  TK_EDEVICE = -5,   /* a HIP call failed; tk_last_hip_error() has its code        */
+  TK_ENOTSUP = -95,  /* combination not provided (e.g. TK_FORM_REF with MFMA)     */
+  TK_ENODEV = -19    /* no gfx950 device visible                                  */
+} tk_status;
+
+typedef enum {
+  TK_METHOD_AUTO = 0,      /* MFMA for canonical/centred forms, butterfly for REF */
+  TK_METHOD_BUTTERFLY = 1, /* scalar Cooley-Tukey / Gentleman-Sande kernel        */
+  TK_METHOD_MFMA = 2,      /* INT8-MFMA matrix formulation, 7-layer exact         */
+  TK_METHOD_MFMA16 = 3,    /* MFMA, v_mfma_i32_16x16x32_i8 tiles (16 polys/wave)  */
+  TK_METHOD_MFMA32 = 4     /* MFMA, v_mfma_i32_32x32x32_i8 tiles (32 polys/wave)  */
+} tk_method;
+
+typedef enum {
+  TK_FORM_DEFAULT = 0,   /* canonical for ntt_batch, centred for invntt_batch     */
+  TK_FORM_CANONICAL = 1, /* [0, q)                                                 */
+  TK_FORM_CENTERED = 2,  /* [-(q-1)/2, (q-1)/2]                                    */
+  TK_FORM_REF = 3        /* int16 values identical to the reference C functions   */
+} tk_form;
+
+/* ---- library state --------------------------------------------------- */
+
+/* Builds the constant tables on the host and uploads them to the current
+ * device.  Idempotent and thread-safe; every entry point calls it. */
+int tk_init(void);
+const char *tk_strerror(int status);
+/* hipError_t of the last failing HIP call on this thread (0 if none). */
+int tk_last_hip_error(void);
+const char *tk_version(void);
+
+/* ---- drop-in batch transforms (device pointers) ------------------------ */
+
+int ntt_batch(int16_t *polys, size_t npoly, void *stream);
+int invntt_batch(int16_t *polys, size_t npoly, void *stream);
+/* r = a o b in the NTT domain; pair (4i,4i+1) uses zetas[64+i] and pair
+ * (4i+2,4i+3) uses -zetas[64+i], the pairing of ntt()'s output order.  The
+ * default form is TK_FORM_REF: int16 values identical to basemul(). */
+int basemul_batch(int16_t *r, const int16_t *a, const int16_t *b, size_t npoly, void *stream);
+
+/* out may equal in (in place) or must not overlap it. */
+int ntt_batch_ex(int16_t *out, const int16_t *in, size_t npoly, int method, int form,
+                 void *stream);
+int invntt_batch_ex(int16_t *out, const int16_t *in, size_t npoly, int method, int form,
+                    void *stream);
+int basemul_batch_ex(int16_t *r, const int16_t *a, const int16_t *b, size_t npoly, int form,
+                     void *stream);
+
+/* ---- ragged batches: one persistent kernel over a request table -------- */
+
+/* One request: npoly polys at base (device pointer), a rank-k module vector
+ * (kind 0, npoly a multiple of k) or k x k matrix (kind 1) of ML-KEM-512/768/
+ * 1024 (k = 2, 3, 4).  k and kind are carried for callers; every poly of every
+ * request is transformed independently. */
+typedef struct {
+  int16_t *base;
+  uint32_t npoly;
+  uint16_t k;
+  uint16_t kind;
+} tk_request;
+
+/* Host-built table of nreq requests; the plan copies it, and its tile map, to
+ * device memory once, picks the kernel shape with the autotuner (or takes
+ * method != AUTO as given) and records the launch in a hipGraph.  direction:
+ * 0 = forward (canonical out), 1 = inverse (centred out).  Transforms are in
+ * place. */
+typedef struct tk_ragged_plan tk_ragged_plan;
+int tk_ragged_plan_create(tk_ragged_plan **plan, const tk_request *table, size_t nreq,
+                          int direction, int method, void *stream);
+/* Replays the captured graph on stream (NULL = the plan's stream). */
+int tk_ragged_plan_execute(tk_ragged_plan *plan, void *stream);
+/* Launches the persistent kernel directly, without the graph. */
+int tk_ragged_plan_launch(tk_ragged_plan *plan, void *stream);
+int tk_ragged_plan_info(const tk_ragged_plan *plan, int *method, int *polys_per_wave,
+                        int *grid, size_t *npoly_total);
+int tk_ragged_plan_destroy(tk_ragged_plan *plan);
+
+/* One-shot form: device request table (nreq entries) already in HBM. */
+int ntt_batch_ragged(const tk_request *table_dev, size_t nreq, size_t npoly_total, int direction,
+                     int method, void *stream);
+
+/* ---- fused module matrix-vector product -------------------------------- */
+
+/* For each of ninst instances: t_i = invntt( sum_j A_hat[i][j] o ntt(s_j) ),
+ * i, j < k (k in 2..4).  A_hat: [ninst][k][k][256] in the NTT domain, s:
+ * [ninst][k][256], t: [ninst][k][256], all device pointers.  One kernel; the
+ * NTT of s and the products stay in registers/LDS.  Output is centred. */
+int matvec_ntt_fused(int16_t *t, const int16_t *A_hat, const int16_t *s, int k, size_t ninst,
+                     void *stream);
+
+/* ---- autotuner ---------------------------------------------------------- */
+
+/* Times each MFMA shape / grid size on scratch data of npoly polys on the
+ * current device and returns the fastest method (TK_METHOD_MFMA16 or
+ * TK_METHOD_MFMA32) and waves per CU; results are cached per device and
+ * direction for the life of the process. */
+int tk_autotune(size_t npoly, int direction, int *best_method, int *best_waves_per_cu,
+                void *stream);
+
+/* ---- introspection (host only, no device needed) ----------------------- */
+
+/* Copies the host-built tables: zetas[128] (Montgomery form, centred, as the
+ * reference table), and the plain mod-q stage matrices: fwd1[16*16]
+ * (layers 1-4, [row][m]), fwd2[16*16*16] (layers 5-7, [block][c][j]),
+ * inv2[16*16*16] (inverse layers 7-5, [block][c][j]) and inv1[16*16]
+ * (inverse layers 4-1 with the final 1441 scaling).  Any pointer may be NULL. */
+int tk_host_tables(int16_t *zetas, int16_t *fwd1, int16_t *fwd2, int16_t *inv2, int16_t *inv1);
+
+/* Copies the packed MFMA A operands uploaded for shape 16 (16x16x32 tiles,
+ * ops = [nmat][2 limbs][64 lanes][2 dwords]) or 32 (32x32x32 tiles,
+ * ops = [nmat][64 lanes][4 dwords]) of stage 0 (forward layers 1-4, nmat 1),
+ * 1 (forward layers 5-7, nmat 16), 2 (inverse layers 7-5, nmat 16) or 3
+ * (inverse layers 4-1, nmat 1), and the row biases bias = [nmat][16]. */
+int tk_host_mfma_operands(int shape, int stage, uint32_t *ops, int32_t *bias);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

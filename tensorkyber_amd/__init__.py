@@ -1,0 +1,293 @@
+"""MI355X batched ML-KEM NTT engine: Python host side over the C ABI.
+
+The product is the native library ``libtkyber.so`` (hand-written gfx950 HIP
+kernels behind ``include/tkyber.h``).  This module loads it with ctypes and
+mirrors the reference interface on device tensors:
+
+=====================  ==================================================
+reference (ntt_bench)  here
+=====================  ==================================================
+``ntt(r)``             ``ntt(polys)``      ref/ntt.h:10-11, ref/ntt.c:80-95
+``invntt(r)``          ``invntt(polys)``   ref/ntt.h:13-14, ref/ntt.c:106-126
+``basemul(r,a,b,z)``   ``basemul(a, b)``   ref/ntt.h:16-17, ref/ntt.c:139-146
+``k_*(int16_t* d, int  the same calls on a [npoly, 256] int16 CUDA tensor
+npoly)``               (ntt_bench.cu:61-62, 97-98, 181-182, 230-231)
+=====================  ==================================================
+
+PyTorch only provides device memory and the current stream; every transform
+runs in the HIP kernels.  There is no CPU fallback: if the library is not
+built, importing the product API raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+N = 256
+Q = 3329
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libtkyber.so")
+
+# tk_status
+OK, EINVAL, ENOMEM, EDEVICE, ENOTSUP, ENODEV = 0, -22, -12, -5, -95, -19
+# tk_method
+METHOD_AUTO, METHOD_BUTTERFLY, METHOD_MFMA, METHOD_MFMA16, METHOD_MFMA32 = 0, 1, 2, 3, 4
+# tk_form
+FORM_DEFAULT, FORM_CANONICAL, FORM_CENTERED, FORM_REF = 0, 1, 2, 3
+
+METHODS = {"auto": METHOD_AUTO, "butterfly": METHOD_BUTTERFLY, "mfma": METHOD_MFMA,
+           "mfma16": METHOD_MFMA16, "mfma32": METHOD_MFMA32}
+FORMS = {"default": FORM_DEFAULT, "canonical": FORM_CANONICAL, "centered": FORM_CENTERED,
+         "ref": FORM_REF}
+
+
+class TkError(RuntimeError):
+    def __init__(self, status: int, what: str, hip: int = 0):
+        self.status = status
+        msg = f"{what}: {_LIB.tk_strerror(status).decode() if _LIB else status}"
+        if status == EDEVICE:
+            msg += f" (hipError {hip})"
+        super().__init__(msg)
+
+
+_i16p = ctypes.POINTER(ctypes.c_int16)
+_LIB = None
+
+
+class TkRequest(ctypes.Structure):
+    _fields_ = [("base", ctypes.c_void_p), ("npoly", ctypes.c_uint32),
+                ("k", ctypes.c_uint16), ("kind", ctypes.c_uint16)]
+
+
+class TkWlRequest(ctypes.Structure):
+    _fields_ = [("offset", ctypes.c_uint64), ("npoly", ctypes.c_uint32),
+                ("k", ctypes.c_uint16), ("kind", ctypes.c_uint16)]
+
+
+def _declare(lib):
+    vp, sz, i, u32, u64 = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint64
+    sigs = {
+        "tk_init": ([], i),
+        "tk_strerror": ([i], ctypes.c_char_p),
+        "tk_last_hip_error": ([], i),
+        "tk_version": ([], ctypes.c_char_p),
+        "ntt_batch": ([vp, sz, vp], i),
+        "invntt_batch": ([vp, sz, vp], i),
+        "basemul_batch": ([vp, vp, vp, sz, vp], i),
+        "ntt_batch_ex": ([vp, vp, sz, i, i, vp], i),
+        "invntt_batch_ex": ([vp, vp, sz, i, i, vp], i),
+        "basemul_batch_ex": ([vp, vp, vp, sz, i, vp], i),
+        "tk_host_tables": ([vp, vp, vp, vp, vp], i),
+        "tk_host_mfma_operands": ([i, i, vp, vp], i),
+        "tk_wl_draw": ([u64, u64, u64], u64),
+        "tk_wl_below": ([u64, u64, u64, u32], u32),
+        "tk_wl_uniform": ([vp, sz, u64, u64, i, i], None),
+        "tk_wl_edge_mix": ([vp, sz, u64], None),
+        "tk_wl_cbd2": ([vp, sz, u64, u64], None),
+        "tk_wl_ragged_layout": ([vp, sz, sz, u64, ctypes.POINTER(sz)], sz),
+        "tk_wl_canon_digest": ([vp, sz], u64),
+    }
+    optional = {
+        "tk_ragged_plan_create": ([ctypes.POINTER(vp), vp, sz, i, i, vp], i),
+        "tk_ragged_plan_execute": ([vp, vp], i),
+        "tk_ragged_plan_launch": ([vp, vp], i),
+        "tk_ragged_plan_info": ([vp, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(i),
+                                 ctypes.POINTER(sz)], i),
+        "tk_ragged_plan_destroy": ([vp], i),
+        "ntt_batch_ragged": ([vp, sz, sz, i, i, vp], i),
+        "matvec_ntt_fused": ([vp, vp, vp, i, sz, vp], i),
+        "tk_autotune": ([sz, i, ctypes.POINTER(i), ctypes.POINTER(i), vp], i),
+    }
+    for name, (args, res) in sigs.items():
+        f = getattr(lib, name)
+        f.argtypes, f.restype = args, res
+    for name, (args, res) in optional.items():
+        if hasattr(lib, name):
+            f = getattr(lib, name)
+            f.argtypes, f.restype = args, res
+    return lib
+
+
+def load_library(path: str = LIB_PATH):
+    """Loads libtkyber.so (raises if it was not built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(path):
+            raise ImportError(f"{path} is missing: build it with `python -m tensorkyber_amd.build`")
+        _LIB = _declare(ctypes.CDLL(path))
+    return _LIB
+
+
+def lib():
+    return load_library()
+
+
+def _check(status: int, what: str):
+    if status != OK:
+        raise TkError(status, what, lib().tk_last_hip_error())
+
+
+# ---------------------------------------------------------------------------
+# device-tensor API (torch is plumbing: memory and the current stream)
+# ---------------------------------------------------------------------------
+
+def _torch():
+    import torch
+    return torch
+
+
+def _stream_ptr(stream=None):
+    torch = _torch()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def _as_polys(t, name="polys"):
+    torch = _torch()
+    if not isinstance(t, torch.Tensor) or not t.is_cuda or t.dtype != torch.int16:
+        raise TypeError(f"{name} must be an int16 CUDA tensor")
+    if not t.is_contiguous() or t.numel() % N:
+        raise ValueError(f"{name} must be contiguous with a multiple of {N} elements")
+    return t
+
+
+def _method(m):
+    return METHODS[m] if isinstance(m, str) else int(m)
+
+
+def _form(f):
+    return FORMS[f] if isinstance(f, str) else int(f)
+
+
+def ntt(polys, out=None, method="auto", form="default", stream=None):
+    """Forward NTT of every poly (in place unless ``out`` is given)."""
+    polys = _as_polys(polys)
+    out = polys if out is None else _as_polys(out, "out")
+    _check(lib().ntt_batch_ex(out.data_ptr(), polys.data_ptr(), polys.numel() // N,
+                              _method(method), _form(form), _stream_ptr(stream)), "ntt_batch")
+    return out
+
+
+def invntt(polys, out=None, method="auto", form="default", stream=None):
+    """Inverse NTT (times 2^16, as invntt_tomont) of every poly."""
+    polys = _as_polys(polys)
+    out = polys if out is None else _as_polys(out, "out")
+    _check(lib().invntt_batch_ex(out.data_ptr(), polys.data_ptr(), polys.numel() // N,
+                                 _method(method), _form(form), _stream_ptr(stream)), "invntt_batch")
+    return out
+
+
+def basemul(a, b, out=None, form="default", stream=None):
+    """NTT-domain product of two poly batches (reference basemul per pair)."""
+    a, b = _as_polys(a, "a"), _as_polys(b, "b")
+    if a.numel() != b.numel():
+        raise ValueError("a and b differ in size")
+    torch = _torch()
+    out = torch.empty_like(a) if out is None else _as_polys(out, "out")
+    _check(lib().basemul_batch_ex(out.data_ptr(), a.data_ptr(), b.data_ptr(), a.numel() // N,
+                                  _form(form), _stream_ptr(stream)), "basemul_batch")
+    return out
+
+
+def matvec(A_hat, s, k: int, out=None, stream=None):
+    """Fused t = invntt(A_hat o ntt(s)) for [ninst, k, k, 256] / [ninst, k, 256]."""
+    A_hat, s = _as_polys(A_hat, "A_hat"), _as_polys(s, "s")
+    ninst = s.numel() // (k * N)
+    if A_hat.numel() != ninst * k * k * N:
+        raise ValueError("A_hat and s shapes disagree")
+    torch = _torch()
+    out = torch.empty_like(s) if out is None else _as_polys(out, "out")
+    _check(lib().matvec_ntt_fused(out.data_ptr(), A_hat.data_ptr(), s.data_ptr(), k, ninst,
+                                  _stream_ptr(stream)), "matvec_ntt_fused")
+    return out
+
+
+# ---------------------------------------------------------------------------
+# host introspection and synthetic workloads (no device needed)
+# ---------------------------------------------------------------------------
+
+def host_tables():
+    L = lib()
+    z = np.zeros(128, np.int16)
+    f1 = np.zeros((16, 16), np.int16)
+    f2 = np.zeros((16, 16, 16), np.int16)
+    i2 = np.zeros((16, 16, 16), np.int16)
+    i1 = np.zeros((16, 16), np.int16)
+    _check(L.tk_host_tables(z.ctypes.data, f1.ctypes.data, f2.ctypes.data, i2.ctypes.data,
+                            i1.ctypes.data), "tk_host_tables")
+    return {"zetas": z, "fwd1": f1, "fwd2": f2, "inv2": i2, "inv1": i1}
+
+
+def host_mfma_operands(shape: int, stage: int):
+    nmat = 16 if stage in (1, 2) else 1
+    if shape == 16:
+        ops = np.zeros((nmat, 2, 64, 2), np.uint32)
+    else:
+        ops = np.zeros((nmat, 64, 4), np.uint32)
+    bias = np.zeros((nmat, 16), np.int32)
+    _check(lib().tk_host_mfma_operands(shape, stage, ops.ctypes.data, bias.ctypes.data),
+           "tk_host_mfma_operands")
+    return ops, bias
+
+
+WL_COEF, WL_CAT, WL_POS, WL_REQ, WL_GAP, WL_MAT, WL_SEC = (
+    0x434F4546, 0x434154, 0x504F53, 0x524551, 0x474150, 0x4D4154, 0x534543)
+
+
+def wl_uniform(n: int, seed: int, lo: int = 0, hi: int = Q, stream: int = WL_COEF) -> np.ndarray:
+    a = np.empty(n, np.int16)
+    lib().tk_wl_uniform(a.ctypes.data, n, seed, stream, lo, hi)
+    return a
+
+
+def wl_edge_mix(npoly: int, seed: int) -> np.ndarray:
+    a = np.empty(npoly * N, np.int16)
+    lib().tk_wl_edge_mix(a.ctypes.data, npoly, seed)
+    return a.reshape(npoly, N)
+
+
+def wl_cbd2(n: int, seed: int, stream: int = WL_SEC) -> np.ndarray:
+    a = np.empty(n, np.int16)
+    lib().tk_wl_cbd2(a.ctypes.data, n, seed, stream)
+    return a
+
+
+def wl_ragged_layout(npoly_total: int, seed: int):
+    L = lib()
+    pool = ctypes.c_size_t(0)
+    n = L.tk_wl_ragged_layout(None, 0, npoly_total, seed, ctypes.byref(pool))
+    arr = (TkWlRequest * n)()
+    L.tk_wl_ragged_layout(ctypes.addressof(arr), n, npoly_total, seed, ctypes.byref(pool))
+    reqs = np.array([(r.offset, r.npoly, r.k, r.kind) for r in arr], dtype=np.int64).reshape(-1, 4)
+    return reqs, int(pool.value)
+
+
+def canon_digest(a: np.ndarray) -> int:
+    a = np.ascontiguousarray(a, dtype=np.int16)
+    return int(lib().tk_wl_canon_digest(a.ctypes.data, a.size))
+
+
+def workload(config: int, npoly: int | None = None):
+    """Synthetic inputs of the benchmark configurations (int16 numpy arrays).
+
+    1: uniform [0, q), seed 1, 65,536 polys; 2: uniform [0, q), seed 2,
+    262,144 polys; 3: edge-value mix, seed 3, 2^20 polys.  ``npoly`` takes the
+    first polys of the same stream (any prefix is identical to the full set).
+    """
+    sizes = {1: 65536, 2: 262144, 3: 1 << 20}
+    n = sizes[config] if npoly is None else npoly
+    if config in (1, 2):
+        return wl_uniform(n * N, seed=config).reshape(n, N)
+    if config == 3:
+        return wl_edge_mix(n, seed=3)
+    raise ValueError(config)
+
+
+__all__ = [
+    "N", "Q", "TkError", "load_library", "lib", "ntt", "invntt", "basemul", "matvec",
+    "host_tables", "host_mfma_operands", "workload", "wl_uniform", "wl_edge_mix", "wl_cbd2",
+    "wl_ragged_layout", "canon_digest", "TkRequest", "METHODS", "FORMS",
+]

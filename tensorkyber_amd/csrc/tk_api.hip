@@ -1,0 +1,319 @@
+// C ABI (include/tkyber.h): argument validation, per-device constant tables,
+// method/form resolution and dispatch to the kernel launchers.
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <mutex>
+
+#include "tk_internal.h"
+#include "tk_launch.h"
+#include "tk_state.h"
+#include "tkyber.h"
+
+namespace tk {
+namespace {
+
+std::mutex g_mu;
+DevState g_dev[kMaxDevices];
+HostMatrices g_hm;
+DeviceTables g_dt;
+bool g_host_built = false;
+thread_local int g_last_hip = 0;
+
+void build_host_once()
+{
+  if (!g_host_built) {
+    build_tables(&g_hm, &g_dt);
+    g_host_built = true;
+  }
+}
+
+}  // namespace
+
+int record_hip(hipError_t e)
+{
+  if (e == hipSuccess)
+    return TK_OK;
+  g_last_hip = (int)e;
+  return TK_EDEVICE;
+}
+
+int after_launch()
+{
+  return record_hip(hipGetLastError());
+}
+
+int init_device(DevState **out)
+{
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) {
+    if (getenv("TK_DEBUG"))
+      fprintf(stderr, "tkyber: hipGetDevice failed: %d\n", (int)e);
+    record_hip(e);
+    return TK_ENODEV;
+  }
+  if (dev < 0 || dev >= kMaxDevices)
+    return TK_ENODEV;
+  std::lock_guard<std::mutex> lk(g_mu);
+  DevState &st = g_dev[dev];
+  if (!st.tables) {
+    hipDeviceProp_t prop;
+    if ((e = hipGetDeviceProperties(&prop, dev)) != hipSuccess) {
+      if (getenv("TK_DEBUG"))
+        fprintf(stderr, "tkyber: hipGetDeviceProperties failed: %d\n", (int)e);
+      record_hip(e);
+      return TK_ENODEV;
+    }
+    if (strstr(prop.gcnArchName, "gfx950") == nullptr) {
+      if (getenv("TK_DEBUG"))
+        fprintf(stderr, "tkyber: device %d is '%s', not gfx950\n", dev, prop.gcnArchName);
+      return TK_ENODEV;
+    }
+    build_host_once();
+    DeviceTables *d = nullptr;
+    if (hipMalloc(&d, sizeof(DeviceTables)) != hipSuccess)
+      return TK_ENOMEM;
+    if ((e = hipMemcpy(d, &g_dt, sizeof(DeviceTables), hipMemcpyHostToDevice)) != hipSuccess) {
+      (void)hipFree(d);
+      return record_hip(e);
+    }
+    st.ncu = prop.multiProcessorCount;
+    st.device = dev;
+    st.tables = d;
+  }
+  *out = &st;
+  return TK_OK;
+}
+
+const HostMatrices &host_matrices()
+{
+  std::lock_guard<std::mutex> lk(g_mu);
+  build_host_once();
+  return g_hm;
+}
+
+const DeviceTables &host_device_tables()
+{
+  std::lock_guard<std::mutex> lk(g_mu);
+  build_host_once();
+  return g_dt;
+}
+
+bool aligned16(const void *p) { return p && ((uintptr_t)p & 15u) == 0; }
+
+bool partial_overlap(const void *a, const void *b, size_t bytes)
+{
+  if (a == b)
+    return false;
+  const uintptr_t x = (uintptr_t)a, y = (uintptr_t)b;
+  return x < y + bytes && y < x + bytes;
+}
+
+int resolve_method(int method, int form, bool inverse, int *out_method, int *out_form)
+{
+  if (form == TK_FORM_DEFAULT)
+    form = inverse ? TK_FORM_CENTERED : TK_FORM_CANONICAL;
+  if (form < TK_FORM_CANONICAL || form > TK_FORM_REF)
+    return TK_EINVAL;
+  if (method == TK_METHOD_AUTO)
+    method = form == TK_FORM_REF ? TK_METHOD_BUTTERFLY : TK_METHOD_MFMA;
+  if (method == TK_METHOD_MFMA)
+    method = TK_METHOD_MFMA16;
+  if (method < TK_METHOD_BUTTERFLY || method > TK_METHOD_MFMA32)
+    return TK_EINVAL;
+  if (form == TK_FORM_REF && method != TK_METHOD_BUTTERFLY)
+    return TK_ENOTSUP;
+  *out_method = method;
+  *out_form = form;
+  return TK_OK;
+}
+
+int device_form(int form)
+{
+  switch (form) {
+    case TK_FORM_CANONICAL: return kFormCanon;
+    case TK_FORM_CENTERED: return kFormCentred;
+    default: return kFormRef;
+  }
+}
+
+int grid_cap(const DevState &st, int method)
+{
+  // Persistent grids: workgroups per CU that fit the LDS / VGPR budget.
+  switch (method) {
+    case TK_METHOD_MFMA16: return st.ncu * 4;
+    case TK_METHOD_MFMA32: return st.ncu * 2;
+    default: return st.ncu * 8;
+  }
+}
+
+int transform(bool inverse, int16_t *out, const int16_t *in, size_t npoly, int method, int form,
+              void *stream)
+{
+  int m = 0, f = 0;
+  int rc = resolve_method(method, form, inverse, &m, &f);
+  if (rc)
+    return rc;
+  if (npoly == 0)
+    return TK_OK;
+  if (!aligned16(out) || !aligned16(in) || npoly > kMaxPolys)
+    return TK_EINVAL;
+  if (partial_overlap(out, in, npoly * kN * sizeof(int16_t)))
+    return TK_EINVAL;
+  DevState *st = nullptr;
+  if ((rc = init_device(&st)))
+    return rc;
+  hipStream_t s = (hipStream_t)stream;
+  if (m == TK_METHOD_BUTTERFLY)
+    rc = launch_butterfly(inverse, device_form(f), out, in, npoly, st->tables,
+                          grid_cap(*st, m), s);
+  else
+    rc = launch_mfma(m == TK_METHOD_MFMA32 ? 32 : 16, inverse, device_form(f), out, in, npoly,
+                     st->tables, grid_cap(*st, m), 4, s);
+  if (rc)
+    return TK_ENOTSUP;
+  return after_launch();
+}
+
+}  // namespace tk
+
+using namespace tk;
+
+extern "C" {
+
+int tk_init(void)
+{
+  DevState *st = nullptr;
+  return init_device(&st);
+}
+
+const char *tk_strerror(int status)
+{
+  switch (status) {
+    case TK_OK: return "ok";
+    case TK_EINVAL: return "invalid argument";
+    case TK_ENOMEM: return "device allocation failed";
+    case TK_EDEVICE: return "HIP runtime error";
+    case TK_ENOTSUP: return "unsupported method/form combination";
+    case TK_ENODEV: return "no gfx950 device";
+    default: return "unknown status";
+  }
+}
+
+int tk_last_hip_error(void) { return g_last_hip; }
+
+const char *tk_version(void) { return "tensorkyber_amd 0.1 (gfx950)"; }
+
+int ntt_batch(int16_t *polys, size_t npoly, void *stream)
+{
+  return transform(false, polys, polys, npoly, TK_METHOD_AUTO, TK_FORM_DEFAULT, stream);
+}
+
+int invntt_batch(int16_t *polys, size_t npoly, void *stream)
+{
+  return transform(true, polys, polys, npoly, TK_METHOD_AUTO, TK_FORM_DEFAULT, stream);
+}
+
+int ntt_batch_ex(int16_t *out, const int16_t *in, size_t npoly, int method, int form,
+                 void *stream)
+{
+  return transform(false, out, in, npoly, method, form, stream);
+}
+
+int invntt_batch_ex(int16_t *out, const int16_t *in, size_t npoly, int method, int form,
+                    void *stream)
+{
+  return transform(true, out, in, npoly, method, form, stream);
+}
+
+int basemul_batch_ex(int16_t *r, const int16_t *a, const int16_t *b, size_t npoly, int form,
+                     void *stream)
+{
+  if (form == TK_FORM_DEFAULT)
+    form = TK_FORM_REF;
+  if (form < TK_FORM_CANONICAL || form > TK_FORM_REF)
+    return TK_EINVAL;
+  if (npoly == 0)
+    return TK_OK;
+  if (!aligned16(r) || !aligned16(a) || !aligned16(b) || npoly > kMaxPolys)
+    return TK_EINVAL;
+  const size_t bytes = npoly * kN * sizeof(int16_t);
+  if (partial_overlap(r, a, bytes) || partial_overlap(r, b, bytes))
+    return TK_EINVAL;
+  DevState *st = nullptr;
+  int rc = init_device(&st);
+  if (rc)
+    return rc;
+  if (launch_basemul(device_form(form), r, a, b, npoly, st->tables, st->ncu * 16,
+                     (hipStream_t)stream))
+    return TK_ENOTSUP;
+  return after_launch();
+}
+
+int basemul_batch(int16_t *r, const int16_t *a, const int16_t *b, size_t npoly, void *stream)
+{
+  return basemul_batch_ex(r, a, b, npoly, TK_FORM_DEFAULT, stream);
+}
+
+int tk_host_tables(int16_t *zetas, int16_t *fwd1, int16_t *fwd2, int16_t *inv2, int16_t *inv1)
+{
+  const HostMatrices &hm = host_matrices();
+  if (zetas)
+    memcpy(zetas, hm.zetas, sizeof(hm.zetas));
+  if (fwd1)
+    memcpy(fwd1, hm.fwd1, sizeof(hm.fwd1));
+  if (fwd2)
+    memcpy(fwd2, hm.fwd2, sizeof(hm.fwd2));
+  if (inv2)
+    memcpy(inv2, hm.inv2, sizeof(hm.inv2));
+  if (inv1)
+    memcpy(inv1, hm.inv1, sizeof(hm.inv1));
+  return TK_OK;
+}
+
+int tk_host_mfma_operands(int shape, int stage, uint32_t *ops, int32_t *bias)
+{
+  if ((shape != 16 && shape != 32) || stage < kFwd1 || stage > kInv1)
+    return TK_EINVAL;
+  const DeviceTables &dt = host_device_tables();
+  const void *src = nullptr;
+  size_t n = 0;
+  const int32_t *b = nullptr;
+  size_t nb = 0;
+  switch (stage) {
+    case kFwd1:
+      src = shape == 16 ? (const void *)dt.m16_fwd1 : (const void *)dt.m32_fwd1;
+      n = shape == 16 ? sizeof(dt.m16_fwd1) : sizeof(dt.m32_fwd1);
+      b = &dt.bias_fwd1[0][0];
+      nb = sizeof(dt.bias_fwd1);
+      break;
+    case kFwd2:
+      src = shape == 16 ? (const void *)dt.m16_fwd2 : (const void *)dt.m32_fwd2;
+      n = shape == 16 ? sizeof(dt.m16_fwd2) : sizeof(dt.m32_fwd2);
+      b = &dt.bias_fwd2[0][0];
+      nb = sizeof(dt.bias_fwd2);
+      break;
+    case kInv2:
+      src = shape == 16 ? (const void *)dt.m16_inv2 : (const void *)dt.m32_inv2;
+      n = shape == 16 ? sizeof(dt.m16_inv2) : sizeof(dt.m32_inv2);
+      b = &dt.bias_inv2[0][0];
+      nb = sizeof(dt.bias_inv2);
+      break;
+    default:
+      src = shape == 16 ? (const void *)dt.m16_inv1 : (const void *)dt.m32_inv1;
+      n = shape == 16 ? sizeof(dt.m16_inv1) : sizeof(dt.m32_inv1);
+      b = &dt.bias_inv1[0][0];
+      nb = sizeof(dt.bias_inv1);
+      break;
+  }
+  if (ops)
+    memcpy(ops, src, n);
+  if (bias)
+    memcpy(bias, b, nb);
+  return TK_OK;
+}
+
+}  // extern "C"

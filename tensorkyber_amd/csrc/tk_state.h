@@ -1,0 +1,34 @@
+// Per-device library state and the validation helpers shared by the C ABI
+// translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "tk_internal.h"
+
+namespace tk {
+
+constexpr int kMaxDevices = 64;
+// Upper bound on one batch: 2^36 polys (32 TiB) keeps every index in size_t
+// arithmetic far from overflow.
+constexpr size_t kMaxPolys = (size_t)1 << 36;
+
+struct DevState {
+  int device = -1;
+  int ncu = 0;
+  DeviceTables *tables = nullptr;
+};
+
+// Uploads the tables to the current device on first use.  Thread-safe.
+int init_device(DevState **out);
+int record_hip(hipError_t e);
+int after_launch();
+bool aligned16(const void *p);
+bool partial_overlap(const void *a, const void *b, size_t bytes);
+int resolve_method(int method, int form, bool inverse, int *out_method, int *out_form);
+int device_form(int form);
+int grid_cap(const DevState &st, int method);
+const HostMatrices &host_matrices();
+const DeviceTables &host_device_tables();
+
+}  // namespace tk

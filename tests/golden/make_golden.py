@@ -1,0 +1,125 @@
+#!/usr/bin/env python3
+"""Writes the golden vectors of the NTT path from the compiled reference.
+
+Runs in the development container only, where the reference checkout exists
+and ``make -C oracle`` has compiled its ``ntt_bench/ref/ntt.c`` and
+``ntt_bench/ref/reduce.c`` into ``oracle/_ref/libkyber_ref.so``.  The
+reference ships no vectors of its own (its only check compares 64 polys drawn
+with glibc rand(), ntt_bench.cu:941-961), so these are produced here:
+
+* ``fixtures.npz``: small inputs and the reference's raw int16 outputs of
+  ntt(), invntt(), invntt(ntt()), whole-poly basemul and the k = 2, 3, 4
+  module matrix-vector product (composed from the reference's functions).
+* ``digests.json``: FNV-1a digests (tk_wl_canon_digest) of the canonical
+  outputs of the full-size benchmark configurations, inputs from the
+  project's SplitMix64 streams (include/tkyber_workload.h).
+
+Usage: python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+import tensorkyber_amd as tk  # noqa: E402  (host-only workload generators)
+from oracle.oracle import Reference, canon  # noqa: E402
+
+N, Q = 256, 3329
+NFIX = 32
+
+
+def fixtures(ref: Reference) -> dict:
+    cats = {
+        "uniform": tk.wl_uniform(NFIX * N, 101).reshape(NFIX, N),
+        "full": tk.wl_uniform(NFIX * N, 102, -32768, 32768).reshape(NFIX, N),
+        "edge": tk.wl_edge_mix(NFIX, 103),
+        "small": tk.wl_uniform(NFIX * N, 104, -16383, 16384).reshape(NFIX, N),
+    }
+    cats["full"][0, :] = 32767
+    cats["full"][1, :] = -32768
+    out = {}
+    for name, x in cats.items():
+        out[f"in_{name}"] = x
+        out[f"ntt_{name}"] = ref.ntt(x)
+        out[f"invntt_{name}"] = ref.invntt(x)
+        out[f"rt_{name}"] = ref.roundtrip(x)
+    a = tk.wl_uniform(NFIX * N, 105, -32768, 32768).reshape(NFIX, N)
+    b = tk.wl_uniform(NFIX * N, 106, -32768, 32768).reshape(NFIX, N)
+    out["bm_a"], out["bm_b"], out["bm_r"] = a, b, ref.basemul(a, b)
+    for k in (2, 3, 4):
+        A = tk.wl_uniform(4 * k * k * N, 107 + k, 0, Q).reshape(4, k, k, N)
+        s = tk.wl_cbd2(4 * k * N, 107 + k).reshape(4, k, N)
+        out[f"mv{k}_A"], out[f"mv{k}_s"], out[f"mv{k}_t"] = A, s, ref.matvec(A, s, k)
+    return out
+
+
+def config_inputs(cfg: int) -> np.ndarray:
+    return tk.workload(cfg)
+
+
+def ragged_pool():
+    reqs, pool = tk.wl_ragged_layout(1 << 20, 4)
+    data = tk.wl_uniform(pool * N, 4).reshape(pool, N)
+    return reqs, data
+
+
+def matvec_inputs(ninst: int = 65536, k: int = 3):
+    A = tk.wl_uniform(ninst * k * k * N, 5, 0, Q, stream=tk.WL_MAT).reshape(ninst, k, k, N)
+    s = tk.wl_cbd2(ninst * k * N, 5, stream=tk.WL_SEC).reshape(ninst, k, N)
+    return A, s
+
+
+def digests(ref: Reference) -> dict:
+    d = {}
+    for cfg in (1, 2, 3):
+        x = config_inputs(cfg)
+        f = ref.ntt(x)
+        inv = ref.invntt(canon(f))
+        rt = ref.invntt(f)
+        d[f"config{cfg}"] = {
+            "npoly": int(x.shape[0]),
+            "input": tk.canon_digest(x),
+            "ntt": tk.canon_digest(f),
+            "invntt_of_canonical_ntt": tk.canon_digest(inv),
+            "invntt_of_raw_ntt": tk.canon_digest(rt),
+            "raw_and_canonical_chains_agree": bool(np.array_equal(canon(inv), canon(rt))),
+        }
+        print(f"config {cfg}: {d[f'config{cfg}']}", flush=True)
+    reqs, pool = ragged_pool()
+    fwd = pool.copy()
+    for off, n, _k, _kind in reqs:
+        fwd[off:off + n] = ref.ntt(pool[off:off + n])
+    inv = fwd.copy()
+    for off, n, _k, _kind in reqs:
+        inv[off:off + n] = ref.invntt(canon(fwd[off:off + n]))
+    d["config4"] = {"nreq": int(len(reqs)), "pool_polys": int(pool.shape[0]),
+                    "npoly": int(reqs[:, 1].sum()), "input": tk.canon_digest(pool),
+                    "ntt": tk.canon_digest(fwd), "invntt_of_canonical_ntt": tk.canon_digest(inv)}
+    print(f"config 4: {d['config4']}", flush=True)
+    A, s = matvec_inputs()
+    t = ref.matvec(A, s, 3)
+    d["config5"] = {"ninst": int(s.shape[0]), "k": 3, "A": tk.canon_digest(A),
+                    "s": tk.canon_digest(s), "t": tk.canon_digest(t)}
+    print(f"config 5: {d['config5']}", flush=True)
+    return d
+
+
+def main():
+    ref = Reference()
+    fx = fixtures(ref)
+    np.savez_compressed(os.path.join(HERE, "fixtures.npz"), **fx)
+    d = digests(ref)
+    with open(os.path.join(HERE, "digests.json"), "w") as f:
+        json.dump(d, f, indent=1, sort_keys=True)
+    print("wrote fixtures.npz and digests.json")
+
+
+if __name__ == "__main__":
+    main()

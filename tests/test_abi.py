@@ -1,0 +1,120 @@
+"""C ABI without a GPU: the library loads, exports every declared symbol, and
+rejects bad arguments before anything touches the device; synthetic workload
+generators are deterministic and have the documented shapes."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+N, Q = 256, 3329
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = []
+    for hdr in ("tkyber.h", "tkyber_workload.h"):
+        text = open(os.path.join(ROOT, "include", hdr)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names += re.findall(r"^[A-Za-z_][\w \*]*?\b(\w+)\s*\([^;]*\)\s*;", text, flags=re.M)
+    return sorted(set(names))
+
+
+def test_exports_every_declared_symbol(tk):
+    names = declared_functions()
+    assert "ntt_batch" in names and "invntt_batch" in names and "basemul_batch" in names
+    lib = ctypes.CDLL(tk.LIB_PATH)
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, f"declared but not exported: {missing}"
+
+
+def test_version_and_strerror(tk):
+    L = tk.lib()
+    assert b"gfx950" in L.tk_version()
+    for code in (0, -22, -12, -5, -95, -19):
+        assert L.tk_strerror(code) and L.tk_strerror(code) != b"unknown status"
+
+
+def test_validation_before_device(tk):
+    L = tk.lib()
+    buf = (ctypes.c_int16 * (N * 8 + 16))()
+    base = ctypes.addressof(buf)
+    al = (base + 15) & ~15
+    assert L.ntt_batch(None, 5, None) == tk.EINVAL
+    assert L.invntt_batch(None, 5, None) == tk.EINVAL
+    assert L.ntt_batch(al + 2, 1, None) == tk.EINVAL            # misaligned
+    assert L.ntt_batch(None, 0, None) == tk.OK                  # empty batch: no-op
+    assert L.ntt_batch_ex(al, al, 1, 99, 0, None) == tk.EINVAL  # bad method
+    assert L.ntt_batch_ex(al, al, 1, 0, 99, None) == tk.EINVAL  # bad form
+    assert L.ntt_batch_ex(al, al, 1, tk.METHOD_MFMA, tk.FORM_REF, None) == tk.ENOTSUP
+    assert L.invntt_batch_ex(al + 32, al, 2, 0, 0, None) == tk.EINVAL  # partial overlap
+    assert L.basemul_batch(None, al, al, 1, None) == tk.EINVAL
+    assert L.basemul_batch_ex(al, al, al, 1, 99, None) == tk.EINVAL
+    assert L.basemul_batch(al + 32, al, al, 2, None) == tk.EINVAL
+
+
+def test_ragged_and_fused_validation(tk):
+    L = tk.lib()
+    if not hasattr(L, "matvec_ntt_fused"):
+        pytest.skip("fused entry point not built")
+    assert L.matvec_ntt_fused(None, None, None, 3, 4, None) == tk.EINVAL
+    assert L.matvec_ntt_fused(None, None, None, 3, 0, None) == tk.OK
+    buf = (ctypes.c_int16 * (N * 64 + 16))()
+    al = (ctypes.addressof(buf) + 15) & ~15
+    assert L.matvec_ntt_fused(al, al, al, 5, 1, None) == tk.EINVAL    # k out of range
+    assert L.ntt_batch_ragged(None, 3, 10, 0, 0, None) == tk.EINVAL
+    plan = ctypes.c_void_p()
+    assert L.tk_ragged_plan_create(ctypes.byref(plan), None, 3, 0, 0, None) == tk.EINVAL
+    assert L.tk_ragged_plan_destroy(None) == tk.EINVAL
+
+
+def test_workload_determinism(tk):
+    a = tk.wl_uniform(1 << 17, 42)
+    b = tk.wl_uniform(1 << 17, 42)
+    assert np.array_equal(a, b) and a.min() >= 0 and a.max() < Q
+    # counter-based: a prefix equals the start of a longer draw
+    assert np.array_equal(tk.wl_uniform(1000, 42), a[:1000])
+    assert not np.array_equal(tk.wl_uniform(1000, 43), a[:1000])
+    c = tk.wl_uniform(100000, 7, -3328, 3329)
+    assert c.min() == -3328 and c.max() == 3328
+    # the documented draw: ((v >> 32) * n) >> 32
+    v = tk.lib().tk_wl_draw(42, tk.WL_COEF, 5)
+    assert a[5] == ((v >> 32) * Q) >> 32
+
+
+def test_edge_mix_categories(tk):
+    x = tk.wl_edge_mix(20000, 3)
+    zero = (x == 0).all(axis=1)
+    hi = (x == 3328).all(axis=1)
+    lo = (x == -3328).all(axis=1)
+    imp = ((x == 1).sum(axis=1) == 1) & ((x == 0).sum(axis=1) == 255)
+    uni = ~(zero | hi | lo | imp)
+    frac = np.array([uni.mean(), zero.mean(), hi.mean(), lo.mean(), imp.mean()])
+    assert np.allclose(frac, [0.70, 0.10, 0.10, 0.05, 0.05], atol=0.015)
+    assert x[uni].min() >= -3328 and x[uni].max() <= 3328
+
+
+def test_cbd2(tk):
+    s = tk.wl_cbd2(1 << 16, 5)
+    counts = np.bincount(s + 2, minlength=5) / s.size
+    assert s.min() >= -2 and s.max() <= 2
+    assert np.allclose(counts, [1 / 16, 4 / 16, 6 / 16, 4 / 16, 1 / 16], atol=0.01)
+
+
+def test_ragged_layout(tk):
+    reqs, pool = tk.wl_ragged_layout(1 << 20, 4)
+    off, n, k, kind = reqs.T
+    assert n.sum() == 1 << 20 and n.min() >= 1 and n.max() <= 4096
+    assert set(np.unique(k)) <= {2, 3, 4} and set(np.unique(kind)) <= {0, 1}
+    assert np.all(off[1:] >= off[:-1] + n[:-1]), "requests overlap"
+    assert off[-1] + n[-1] <= pool
+    obj = np.where(kind == 1, k * k, k)
+    assert np.all(n[:-1] % obj[:-1] == 0), "requests hold whole vectors/matrices"
+    assert 0.18 < (kind == 1).mean() < 0.32
+
+
+def test_canon_digest_is_representative_free(tk):
+    x = tk.wl_uniform(4096, 9, -32768, 32768)
+    y = (x.astype(np.int32) + Q * np.where(x > 0, -3, 3)).astype(np.int16)
+    assert tk.canon_digest(x) == tk.canon_digest(y)
