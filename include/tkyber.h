@@ -112,9 +112,10 @@ typedef struct {
 
 /* Host-built table of nreq requests; the plan copies it, and its tile map, to
  * device memory once, picks the kernel shape with the autotuner (or takes
- * method != AUTO as given) and records the launch in a hipGraph.  direction:
- * 0 = forward (canonical out), 1 = inverse (centred out).  Transforms are in
- * place. */
+ * method MFMA16 / MFMA32 as given) and records the launch in a hipGraph.
+ * direction: 0 = forward (canonical out), 1 = inverse (centred out).
+ * Transforms are in place.  Plan creation allocates and synchronises; the
+ * execute and launch calls do neither.  Requests must not overlap. */
 typedef struct tk_ragged_plan tk_ragged_plan;
 int tk_ragged_plan_create(tk_ragged_plan **plan, const tk_request *table, size_t nreq,
                           int direction, int method, void *stream);
@@ -122,13 +123,11 @@ int tk_ragged_plan_create(tk_ragged_plan **plan, const tk_request *table, size_t
 int tk_ragged_plan_execute(tk_ragged_plan *plan, void *stream);
 /* Launches the persistent kernel directly, without the graph. */
 int tk_ragged_plan_launch(tk_ragged_plan *plan, void *stream);
+/* Shape the plan runs: method (MFMA16 / MFMA32), polys per wave tile, grid
+ * size in workgroups, and the total number of polys. */
 int tk_ragged_plan_info(const tk_ragged_plan *plan, int *method, int *polys_per_wave,
                         int *grid, size_t *npoly_total);
 int tk_ragged_plan_destroy(tk_ragged_plan *plan);
-
-/* One-shot form: device request table (nreq entries) already in HBM. */
-int ntt_batch_ragged(const tk_request *table_dev, size_t nreq, size_t npoly_total, int direction,
-                     int method, void *stream);
 
 /* ---- fused module matrix-vector product -------------------------------- */
 

@@ -97,7 +97,6 @@ def _declare(lib):
         "tk_ragged_plan_info": ([vp, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(i),
                                  ctypes.POINTER(sz)], i),
         "tk_ragged_plan_destroy": ([vp], i),
-        "ntt_batch_ragged": ([vp, sz, sz, i, i, vp], i),
         "matvec_ntt_fused": ([vp, vp, vp, i, sz, vp], i),
         "tk_autotune": ([sz, i, ctypes.POINTER(i), ctypes.POINTER(i), vp], i),
     }
@@ -117,6 +116,14 @@ def load_library(path: str = LIB_PATH):
     if _LIB is None:
         if not os.path.exists(path):
             raise ImportError(f"{path} is missing: build it with `python -m tensorkyber_amd.build`")
+        # PyTorch-ROCm ships its own libamdhip64.so, which its libraries request
+        # by the unversioned name.  Loaded first, it satisfies this library's
+        # libamdhip64.so.7 too; loaded after this library, it would become a
+        # second HIP runtime in the process and device initialisation fails.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         _LIB = _declare(ctypes.CDLL(path))
     return _LIB
 

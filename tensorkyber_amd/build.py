@@ -30,6 +30,8 @@ HIP_FLAGS = COMMON + [
     f"--offload-arch={ARCH}",
     # keep hipcc from forming packed fp32 instructions on its own
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # MFMA results straight into VGPRs (no AGPR round trip before the epilogue)
+    "-mllvm", "-amdgpu-mfma-vgpr-form",
 ]
 CXX_FLAGS = COMMON + ["-pthread"]
 

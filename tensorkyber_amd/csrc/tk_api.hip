@@ -258,6 +258,30 @@ int basemul_batch(int16_t *r, const int16_t *a, const int16_t *b, size_t npoly, 
   return basemul_batch_ex(r, a, b, npoly, TK_FORM_DEFAULT, stream);
 }
 
+int matvec_ntt_fused(int16_t *t, const int16_t *A_hat, const int16_t *s, int k, size_t ninst,
+                     void *stream)
+{
+  if (k < 2 || k > 4)
+    return TK_EINVAL;
+  if (ninst == 0)
+    return TK_OK;
+  if (!aligned16(t) || !aligned16(A_hat) || !aligned16(s) || ninst > kMaxPolys / 16)
+    return TK_EINVAL;
+  const size_t vb = ninst * k * kN * sizeof(int16_t), mb = vb * k;
+  // t may alias s exactly (each tile reads all of its s before writing t) but
+  // must not overlap A_hat or straddle s.
+  const uintptr_t tp = (uintptr_t)t, ap = (uintptr_t)A_hat, sp = (uintptr_t)s;
+  if ((tp < ap + mb && ap < tp + vb) || (tp != sp && tp < sp + vb && sp < tp + vb))
+    return TK_EINVAL;
+  DevState *st = nullptr;
+  int rc = init_device(&st);
+  if (rc)
+    return rc;
+  if (launch_matvec(k, t, A_hat, s, ninst, st->tables, st->ncu * 3, (hipStream_t)stream))
+    return TK_ENOTSUP;
+  return after_launch();
+}
+
 int tk_host_tables(int16_t *zetas, int16_t *fwd1, int16_t *fwd2, int16_t *inv2, int16_t *inv1)
 {
   const HostMatrices &hm = host_matrices();

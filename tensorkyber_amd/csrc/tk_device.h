@@ -56,6 +56,15 @@ __device__ __forceinline__ uint32_t mulhi_u24(uint32_t a, uint32_t b)
   return r;
 }
 
+// v_mad_i32_i24: a * b + c with a, b taken as signed 24-bit values.  Written
+// out so the subtract of the Barrett step stays one full-rate instruction.
+__device__ __forceinline__ uint32_t mad_i24(uint32_t a, int b, uint32_t c)
+{
+  uint32_t r;
+  asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "s"(b), "v"(c));
+  return r;
+}
+
 // Digits of four int16 values held as two packed pairs d0 = (x0,x1),
 // d1 = (x2,x3): x = 128 + x0' + 256*x1' with x0' = (x & 255) - 128 (byte 0
 // with its top bit flipped) and x1' = x >> 8 (byte 1).  Returns the 8-byte
@@ -86,7 +95,7 @@ __device__ __forceinline__ uint32_t reduce_2q(int lo, int hi)
 {
   const uint32_t s = (uint32_t)lo + ((uint32_t)hi << 8);
   const uint32_t t = mulhi_u24(s, kBarrettM);
-  return s - __umul24(t, (uint32_t)kQ);
+  return mad_i24(t, -kQ, s);
 }
 
 __device__ __forceinline__ uint32_t reduce_q(int lo, int hi)

@@ -54,6 +54,9 @@ struct DeviceTables {
   uint32_t m16_fwd2[16][2][64][2];
   uint32_t m16_inv2[16][2][64][2];
   uint32_t m16_inv1[1][2][64][2];
+  // Inverse layers 4-1 without the Montgomery factor: plain NTT^-1 =
+  // inv1 * 2^-16, used where the input carries no 2^-16 (fused products).
+  uint32_t m16_inv1p[1][2][64][2];
   // MFMA32 operands: [matrix][lane][4 dwords]
   uint32_t m32_fwd1[1][64][4];
   uint32_t m32_fwd2[16][64][4];
@@ -64,6 +67,10 @@ struct DeviceTables {
   int32_t bias_fwd2[16][16];
   int32_t bias_inv2[16][16];
   int32_t bias_inv1[1][16];
+  int32_t bias_inv1p[1][16];  // includes kCentre
+  // Plain-domain basemul twiddles gamma_i = zetas[64+i] * 2^-16 mod q in [0, q)
+  // (pair (4i,4i+1) uses gamma_i, pair (4i+2,4i+3) uses q - gamma_i).
+  uint16_t gamma[64];
 };
 
 // Host-side tables (plain mod-q matrices) kept for introspection.
@@ -73,6 +80,7 @@ struct HostMatrices {
   uint16_t fwd2[16][16][16];  // [block][c][j]: layers 5-7
   uint16_t inv2[16][16][16];  // [block][c][j]: inverse layers len 2,4,8
   uint16_t inv1[16][16];      // [row i][m]   : inverse layers len 16..128, x1441
+  uint16_t inv1p[16][16];     // inv1 * 2^-16 mod q: plain inverse of layers 1-4
 };
 
 // Builds both table sets from closed forms (tk_tables.cpp).  Pure host code.

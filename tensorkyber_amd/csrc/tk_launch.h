@@ -9,11 +9,34 @@
 
 namespace tk {
 
+// A contiguous batch: poly p at in + p*256 (read) and out + p*256 (written).
+struct LinearTiles {
+  const int16_t *in;
+  int16_t *out;
+  size_t npoly;
+};
+
+// A ragged batch of requests, transformed in place: request r holds polys
+// first[r] .. first[r+1]-1 of the global numbering at base[r]; tile_req[t]
+// is the request holding global poly t*P (P = polys per wave tile).
+struct RaggedTiles {
+  int16_t *const *base;
+  const uint64_t *first;
+  const uint32_t *tile_req;
+  size_t npoly;
+};
+
 int launch_butterfly(bool inverse, int form, int16_t *out, const int16_t *in, size_t npoly,
                      const DeviceTables *tab, int grid_cap, hipStream_t stream);
 
 int launch_mfma(int shape, bool inverse, int form, int16_t *out, const int16_t *in, size_t npoly,
                 const DeviceTables *tab, int grid_cap, int waves, hipStream_t stream);
+
+int launch_mfma_ragged(int shape, bool inverse, int form, const RaggedTiles &io,
+                       const DeviceTables *tab, int grid_cap, int waves, hipStream_t stream);
+
+int launch_matvec(int k, int16_t *t, const int16_t *A, const int16_t *s, size_t ninst,
+                  const DeviceTables *tab, int grid_cap, hipStream_t stream);
 
 int launch_basemul(int form, int16_t *r, const int16_t *a, const int16_t *b, size_t npoly,
                    const DeviceTables *tab, int grid_cap, hipStream_t stream);

@@ -1,0 +1,252 @@
+// Device building blocks of the INT8-MFMA transform, shared by the batch,
+// ragged and fused kernels: tile staging between HBM and a per-wave LDS
+// slab, and the two contraction passes for the 16x16x32 and 32x32x32 tile
+// shapes.  See tk_mfma.hip for the formulation.
+#pragma once
+
+#include "tk_device.h"
+
+namespace tk {
+
+constexpr int kStride = 264;  // int16 per poly slot in LDS (512 + 16 bytes)
+
+// ---------------------------------------------------------------------------
+// Tile staging: P polys, coalesced 16-byte global accesses, one LDS slab.
+// `src` yields the address of tile poly pp (nullptr = beyond the batch).
+// ---------------------------------------------------------------------------
+template <int P, class Addr>
+__device__ __forceinline__ void tile_in(int16_t *L, Addr src, int lane)
+{
+  v4i v[P / 2];
+#pragma unroll
+  for (int it = 0; it < P / 2; it++) {
+    const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
+    const int16_t *p = src(pp);
+    v[it] = p ? reinterpret_cast<const v4i *>(p)[ch] : v4i{0, 0, 0, 0};
+  }
+#pragma unroll
+  for (int it = 0; it < P / 2; it++) {
+    const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
+    *reinterpret_cast<v4i *>(L + pp * kStride + ch * 8) = v[it];
+  }
+}
+
+template <int P, class Addr>
+__device__ __forceinline__ void tile_out(const int16_t *L, Addr dst, int lane)
+{
+#pragma unroll
+  for (int it = 0; it < P / 2; it++) {
+    const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
+    const v4i v = *reinterpret_cast<const v4i *>(L + pp * kStride + ch * 8);
+    int16_t *p = dst(pp);
+    if (p)
+      reinterpret_cast<v4i *>(p)[ch] = v;
+  }
+}
+
+typedef unsigned short us2 __attribute__((ext_vector_type(2)));
+
+// Final form of two packed values in [0, 2q): kFormInter keeps them,
+// kFormCanon maps to [0, q), kFormCentred (whose bias carried +kCentre) to
+// [-(q-1)/2, (q-1)/2]; packed 16-bit arithmetic, one pair per instruction.
+template <int MODE>
+__device__ __forceinline__ uint32_t finish2(uint32_t w)
+{
+  if (MODE == kFormInter)
+    return w;
+  us2 v = __builtin_bit_cast(us2, w);
+  const us2 qq = {(unsigned short)kQ, (unsigned short)kQ};
+  v = __builtin_elementwise_min(v, (us2)(v - qq));  // u16 wrap: v - q is large when v < q
+  if (MODE == kFormCentred) {
+    const us2 cc = {(unsigned short)kCentre, (unsigned short)kCentre};
+    v = v - cc;
+  }
+  return __builtin_bit_cast(uint32_t, v);
+}
+
+// Four rows of four int16 each (R[r] = row r as two packed pairs: columns
+// (0,1) and (2,3)) -> the 8-byte MFMA operand of each column: [x0' of rows
+// 0..3 | x1' of rows 0..3], i.e. the digits of a column with its K index
+// running over the rows.  A byte transpose in registers (v_perm_b32).
+__device__ __forceinline__ void col_digits4(const uint2 R[4], long b[4])
+{
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    const uint32_t d0 = h ? R[0].y : R[0].x, d1 = h ? R[1].y : R[1].x;
+    const uint32_t d2 = h ? R[2].y : R[2].x, d3 = h ? R[3].y : R[3].x;
+    const uint32_t l01 = __builtin_amdgcn_perm(d1, d0, 0x06020400u);
+    const uint32_t l23 = __builtin_amdgcn_perm(d3, d2, 0x06020400u);
+    const uint32_t h01 = __builtin_amdgcn_perm(d1, d0, 0x07030501u);
+    const uint32_t h23 = __builtin_amdgcn_perm(d3, d2, 0x07030501u);
+    const uint32_t lo0 = __builtin_amdgcn_perm(l23, l01, 0x05040100u) ^ 0x80808080u;
+    const uint32_t lo1 = __builtin_amdgcn_perm(l23, l01, 0x07060302u) ^ 0x80808080u;
+    const uint32_t hi0 = __builtin_amdgcn_perm(h23, h01, 0x05040100u);
+    const uint32_t hi1 = __builtin_amdgcn_perm(h23, h01, 0x07060302u);
+    b[2 * h] = (long)(((uint64_t)hi0 << 32) | lo0);
+    b[2 * h + 1] = (long)(((uint64_t)hi1 << 32) | lo1);
+  }
+}
+
+__device__ __forceinline__ uint2 ld2(const int16_t *p) { return *reinterpret_cast<const uint2 *>(p); }
+__device__ __forceinline__ void st2(int16_t *p, uint32_t a, uint32_t b)
+{
+  *reinterpret_cast<uint2 *>(p) = make_uint2(a, b);
+}
+
+// ---------------------------------------------------------------------------
+// 16x16x32 passes: lane l = 16g + t works on poly t of the tile; its K group
+// and its output rows are the index set {4g, .., 4g+3}.  Each pass works on
+// four columns / blocks at a time so eight independent MFMAs are in flight
+// before their results are read.
+// ---------------------------------------------------------------------------
+
+// Contraction over m for each column c (one matrix for all columns).
+template <int MODE>
+__device__ __forceinline__ void pass_m16(int16_t *P, int g, long alo, long ahi, v4i bias)
+{
+  const v4i zero = {0, 0, 0, 0};
+  int16_t *rows = P + 64 * g;  // row 4g; row 4g + r at +16r
+#pragma unroll
+  for (int cq = 0; cq < 4; cq++) {
+    uint2 R[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+      R[r] = ld2(rows + 16 * r + 4 * cq);
+    long b[4];
+    col_digits4(R, b);
+    v4i lo[4], hi[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      lo[u] = mfma16(alo, b[u], bias);
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      hi[u] = mfma16(ahi, b[u], zero);
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      uint32_t v[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        v[u] = reduce_2q(lo[u][r], hi[u][r]);
+      st2(rows + 16 * r + 4 * cq, finish2<MODE>(pack2(v[0], v[1])),
+          finish2<MODE>(pack2(v[2], v[3])));
+    }
+  }
+}
+
+// Contraction over c inside each block m (one matrix per block).
+template <int MODE, class ALo, class AHi>
+__device__ __forceinline__ void pass_c16(int16_t *P, int g, ALo alo, AHi ahi,
+                                         const int32_t (*s_bias)[16])
+{
+  const v4i zero = {0, 0, 0, 0};
+#pragma unroll
+  for (int m0 = 0; m0 < 16; m0 += 4) {
+    long b[4];
+    v4i bias[4], lo[4], hi[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const uint2 d = ld2(P + 16 * (m0 + u) + 4 * g);
+      b[u] = digits8(d.x, d.y);
+      bias[u] = *reinterpret_cast<const v4i *>(&s_bias[m0 + u][4 * g]);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      lo[u] = mfma16(alo(m0 + u), b[u], bias[u]);
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      hi[u] = mfma16(ahi(m0 + u), b[u], zero);
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      uint32_t r[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+        r[i] = reduce_2q(lo[u][i], hi[u][i]);
+      st2(P + 16 * (m0 + u) + 4 * g, finish2<MODE>(pack2(r[0], r[1])),
+          finish2<MODE>(pack2(r[2], r[3])));
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 32x32x32 passes: lane l = 32h + t works on poly t; its K group and output
+// rows are U_h = {4h..4h+3, 8+4h..8+4h+3}; accumulator register q (< 8) is
+// the lo limb of row U_h[q], register q+8 its hi limb.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ v16i acc_init(const v4i &b0, const v4i &b1)
+{
+  return v16i{b0[0], b0[1], b0[2], b0[3], b1[0], b1[1], b1[2], b1[3], 0, 0, 0, 0, 0, 0, 0, 0};
+}
+
+__device__ __forceinline__ v4i op32(long a, long b)
+{
+  // [x0' rows U[0..3] | x0' rows U[4..7] | x1' rows U[0..3] | x1' rows U[4..7]]
+  return v4i{(int)(uint32_t)a, (int)(uint32_t)b, (int)(uint32_t)((uint64_t)a >> 32),
+             (int)(uint32_t)((uint64_t)b >> 32)};
+}
+
+template <int MODE>
+__device__ __forceinline__ void pass_m32(int16_t *P, int h, v4i a, const v4i &b0, const v4i &b1)
+{
+  int16_t *ra = P + 64 * h;       // rows 4h .. 4h+3
+  int16_t *rb = P + 128 + 64 * h; // rows 8+4h .. 8+4h+3
+  const v16i acc0 = acc_init(b0, b1);
+#pragma unroll
+  for (int cq = 0; cq < 4; cq++) {
+    uint2 RA[4], RB[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      RA[r] = ld2(ra + 16 * r + 4 * cq);
+      RB[r] = ld2(rb + 16 * r + 4 * cq);
+    }
+    long ba[4], bb[4];
+    col_digits4(RA, ba);
+    col_digits4(RB, bb);
+    v16i acc[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      acc[u] = mfma32(a, op32(ba[u], bb[u]), acc0);
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      uint32_t va[4], vb[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        va[u] = reduce_2q(acc[u][r], acc[u][r + 8]);
+        vb[u] = reduce_2q(acc[u][r + 4], acc[u][r + 12]);
+      }
+      st2(ra + 16 * r + 4 * cq, finish2<MODE>(pack2(va[0], va[1])),
+          finish2<MODE>(pack2(va[2], va[3])));
+      st2(rb + 16 * r + 4 * cq, finish2<MODE>(pack2(vb[0], vb[1])),
+          finish2<MODE>(pack2(vb[2], vb[3])));
+    }
+  }
+}
+
+template <int MODE, class A>
+__device__ __forceinline__ void pass_c32(int16_t *P, int h, A a, const int32_t (*s_bias)[16])
+{
+#pragma unroll
+  for (int m0 = 0; m0 < 16; m0 += 4) {
+    v16i acc[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int m = m0 + u;
+      const uint2 x0 = ld2(P + 16 * m + 4 * h), x1 = ld2(P + 16 * m + 8 + 4 * h);
+      const v4i c0 = *reinterpret_cast<const v4i *>(&s_bias[m][4 * h]);
+      const v4i c1 = *reinterpret_cast<const v4i *>(&s_bias[m][8 + 4 * h]);
+      acc[u] = mfma32(a(m), op32(digits8(x0.x, x0.y), digits8(x1.x, x1.y)), acc_init(c0, c1));
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int m = m0 + u;
+      uint32_t r[8];
+#pragma unroll
+      for (int q = 0; q < 8; q++)
+        r[q] = reduce_2q(acc[u][q], acc[u][q + 8]);
+      st2(P + 16 * m + 4 * h, finish2<MODE>(pack2(r[0], r[1])), finish2<MODE>(pack2(r[2], r[3])));
+      st2(P + 16 * m + 8 + 4 * h, finish2<MODE>(pack2(r[4], r[5])),
+          finish2<MODE>(pack2(r[6], r[7])));
+    }
+  }
+}
+
+}  // namespace tk

@@ -1,0 +1,339 @@
+// Ragged batches (many uneven ML-KEM-512/768/1024 vectors and matrices in
+// one launch) and the launch-shape autotuner.
+//
+// A plan flattens the request table into a global poly numbering, uploads
+// the table (bases, prefix sums) and a tile -> first-request map once, picks
+// the MFMA tile shape / waves per workgroup / grid with the autotuner, and
+// records the persistent kernel into a hipGraph.  Executing the plan replays
+// the graph: one launch whatever the number of requests, no host work.
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "tk_launch.h"
+#include "tk_state.h"
+#include "tkyber.h"
+
+struct tk_ragged_plan {
+  int device = -1;
+  int direction = 0;
+  int shape = 16;
+  int waves = 4;
+  int grid = 0;
+  size_t npoly = 0;
+  size_t nreq = 0;
+  void *dmem = nullptr;
+  tk::RaggedTiles tiles{};
+  const tk::DeviceTables *tab = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace tk {
+namespace {
+
+struct TuneKey {
+  int device, direction, bucket;
+  bool operator<(const TuneKey &o) const
+  {
+    return std::tie(device, direction, bucket) < std::tie(o.device, o.direction, o.bucket);
+  }
+};
+struct TuneVal {
+  int method, waves, wgs_per_cu;
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, TuneVal> g_tune;
+
+int bucket_of(size_t npoly)
+{
+  int b = 0;
+  while (b < 40 && ((size_t)1 << (b + 1)) <= npoly)
+    b++;
+  return b;
+}
+
+// Workgroups per CU that fit in 160 KiB of LDS for a shape / waves choice.
+int max_wgs_per_cu(int shape, int waves)
+{
+  const int lds = waves * shape * 528 + 1024 + waves * shape * 8;
+  int n = (160 * 1024) / lds;
+  return n < 1 ? 1 : (n > 8 ? 8 : n);
+}
+
+int time_config(DevState *st, int shape, int waves, int wgs, bool inverse, int16_t *buf,
+                size_t npoly, hipStream_t s, float *ms_out)
+{
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess)
+    return TK_EDEVICE;
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return TK_EDEVICE;
+  }
+  const int form = inverse ? kFormCentred : kFormCanon;
+  const int grid = st->ncu * wgs;
+  float best = 1e30f;
+  int rc = TK_OK;
+  for (int rep = 0; rep < 4 && rc == TK_OK; rep++) {
+    (void)hipEventRecord(e0, s);
+    launch_mfma(shape, inverse, form, buf, buf, npoly, st->tables, grid, waves, s);
+    (void)hipEventRecord(e1, s);
+    if ((rc = record_hip(hipEventSynchronize(e1))) != TK_OK)
+      break;
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    if (rep > 0 && ms < best)  // first launch is a warm-up
+      best = ms;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms_out = best;
+  return rc;
+}
+
+int autotune(DevState *st, size_t npoly, int direction, TuneVal *out, hipStream_t s)
+{
+  const TuneKey key{st->device, direction, bucket_of(npoly)};
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune.find(key);
+    if (it != g_tune.end()) {
+      *out = it->second;
+      return TK_OK;
+    }
+  }
+  // Time on scratch data of the same size (capped at 2^20 polys); the
+  // kernels' cost does not depend on the values.
+  const size_t n = npoly < ((size_t)1 << 20) ? npoly : ((size_t)1 << 20);
+  int16_t *buf = nullptr;
+  if (hipMalloc(&buf, n * kN * sizeof(int16_t)) != hipSuccess)
+    return TK_ENOMEM;
+  int rc = record_hip(hipMemsetAsync(buf, 0, n * kN * sizeof(int16_t), s));
+  TuneVal best{TK_METHOD_MFMA16, 4, 4};
+  float best_ms = 1e30f;
+  const int shapes[2] = {16, 32};
+  const int wavess[2] = {2, 4};
+  const int wgss[4] = {1, 2, 4, 8};
+  for (int si = 0; si < 2 && rc == TK_OK; si++)
+    for (int wi = 0; wi < 2 && rc == TK_OK; wi++)
+      for (int gi = 0; gi < 4 && rc == TK_OK; gi++) {
+        const int shape = shapes[si], waves = wavess[wi], wgs = wgss[gi];
+        if (wgs > max_wgs_per_cu(shape, waves))
+          continue;
+        float ms = 0;
+        rc = time_config(st, shape, waves, wgs, direction != 0, buf, n, s, &ms);
+        if (rc == TK_OK && ms < best_ms) {
+          best_ms = ms;
+          best = TuneVal{shape == 32 ? TK_METHOD_MFMA32 : TK_METHOD_MFMA16, waves, wgs};
+        }
+      }
+  (void)hipStreamSynchronize(s);
+  (void)hipFree(buf);
+  if (rc != TK_OK)
+    return rc;
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune[key] = best;
+  *out = best;
+  return TK_OK;
+}
+
+void plan_free(tk_ragged_plan *p)
+{
+  if (p->exec)
+    (void)hipGraphExecDestroy(p->exec);
+  if (p->graph)
+    (void)hipGraphDestroy(p->graph);
+  if (p->dmem)
+    (void)hipFree(p->dmem);
+  delete p;
+}
+
+int plan_launch(tk_ragged_plan *p, hipStream_t s)
+{
+  const int form = p->direction ? kFormCentred : kFormCanon;
+  if (launch_mfma_ragged(p->shape, p->direction != 0, form, p->tiles, p->tab, p->grid, p->waves,
+                         s))
+    return TK_ENOTSUP;
+  return after_launch();
+}
+
+}  // namespace
+}  // namespace tk
+
+using namespace tk;
+
+extern "C" {
+
+int tk_autotune(size_t npoly, int direction, int *best_method, int *best_waves_per_cu,
+                void *stream)
+{
+  if (npoly == 0 || (direction != 0 && direction != 1) || npoly > kMaxPolys)
+    return TK_EINVAL;
+  DevState *st = nullptr;
+  int rc = init_device(&st);
+  if (rc)
+    return rc;
+  TuneVal v{};
+  if ((rc = autotune(st, npoly, direction, &v, (hipStream_t)stream)))
+    return rc;
+  if (best_method)
+    *best_method = v.method;
+  if (best_waves_per_cu)
+    *best_waves_per_cu = v.waves * v.wgs_per_cu;
+  return TK_OK;
+}
+
+int tk_ragged_plan_create(tk_ragged_plan **plan, const tk_request *table, size_t nreq,
+                          int direction, int method, void *stream)
+{
+  if (!plan || !table || nreq == 0 || nreq > 0xffffffffu || (direction != 0 && direction != 1))
+    return TK_EINVAL;
+  if (method != TK_METHOD_AUTO && method != TK_METHOD_MFMA && method != TK_METHOD_MFMA16 &&
+      method != TK_METHOD_MFMA32)
+    return TK_EINVAL;
+  *plan = nullptr;
+  size_t total = 0;
+  for (size_t r = 0; r < nreq; r++) {
+    const tk_request &q = table[r];
+    if (q.k < 2 || q.k > 4 || q.kind > 1)
+      return TK_EINVAL;
+    if (q.npoly && !aligned16(q.base))
+      return TK_EINVAL;
+    total += q.npoly;
+    if (total > kMaxPolys)
+      return TK_EINVAL;
+  }
+  if (total == 0)
+    return TK_EINVAL;
+  DevState *st = nullptr;
+  int rc = init_device(&st);
+  if (rc)
+    return rc;
+  hipStream_t s = (hipStream_t)stream;
+
+  TuneVal tv{TK_METHOD_MFMA16, 4, 4};
+  if (method == TK_METHOD_AUTO) {
+    if ((rc = autotune(st, total, direction, &tv, s)))
+      return rc;
+  } else {
+    tv.method = method == TK_METHOD_MFMA32 ? TK_METHOD_MFMA32 : TK_METHOD_MFMA16;
+    tv.waves = 4;
+    tv.wgs_per_cu = tv.method == TK_METHOD_MFMA32 ? 2 : 4;
+  }
+  const int shape = tv.method == TK_METHOD_MFMA32 ? 32 : 16;
+
+  // Host images of the device tables.
+  const size_t ntiles = (total + shape - 1) / shape;
+  std::vector<int16_t *> base(nreq);
+  std::vector<uint64_t> first(nreq + 1);
+  std::vector<uint32_t> tile_req(ntiles);
+  uint64_t acc = 0;
+  for (size_t r = 0; r < nreq; r++) {
+    base[r] = table[r].base;
+    first[r] = acc;
+    acc += table[r].npoly;
+  }
+  first[nreq] = acc;
+  size_t r = 0;
+  for (size_t t = 0; t < ntiles; t++) {
+    const uint64_t g = (uint64_t)t * shape;
+    while (first[r + 1] <= g)
+      r++;
+    tile_req[t] = (uint32_t)r;
+  }
+  const size_t b_base = nreq * sizeof(int16_t *);
+  const size_t b_first = (nreq + 1) * sizeof(uint64_t);
+  const size_t b_tile = ntiles * sizeof(uint32_t);
+  const size_t off_first = (b_base + 255) & ~(size_t)255;
+  const size_t off_tile = (off_first + b_first + 255) & ~(size_t)255;
+
+  tk_ragged_plan *p = new tk_ragged_plan();
+  p->device = st->device;
+  p->direction = direction;
+  p->shape = shape;
+  p->waves = tv.waves;
+  p->grid = st->ncu * tv.wgs_per_cu;
+  p->npoly = total;
+  p->nreq = nreq;
+  p->tab = st->tables;
+  if (hipMalloc(&p->dmem, off_tile + b_tile) != hipSuccess) {
+    plan_free(p);
+    return TK_ENOMEM;
+  }
+  char *d = (char *)p->dmem;
+  if ((rc = record_hip(hipMemcpy(d, base.data(), b_base, hipMemcpyHostToDevice))) ||
+      (rc = record_hip(hipMemcpy(d + off_first, first.data(), b_first, hipMemcpyHostToDevice))) ||
+      (rc = record_hip(hipMemcpy(d + off_tile, tile_req.data(), b_tile, hipMemcpyHostToDevice)))) {
+    plan_free(p);
+    return rc;
+  }
+  p->tiles = RaggedTiles{(int16_t *const *)d, (const uint64_t *)(d + off_first),
+                         (const uint32_t *)(d + off_tile), total};
+
+  // Record the launch into a graph on a private capture stream.
+  hipStream_t cap = nullptr;
+  if ((rc = record_hip(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking)))) {
+    plan_free(p);
+    return rc;
+  }
+  rc = record_hip(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+  if (rc == TK_OK) {
+    const int lrc = plan_launch(p, cap);
+    const int erc = record_hip(hipStreamEndCapture(cap, &p->graph));
+    rc = lrc ? lrc : erc;
+  }
+  if (rc == TK_OK)
+    rc = record_hip(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
+  (void)hipStreamDestroy(cap);
+  if (rc) {
+    plan_free(p);
+    return rc;
+  }
+  *plan = p;
+  return TK_OK;
+}
+
+int tk_ragged_plan_execute(tk_ragged_plan *plan, void *stream)
+{
+  if (!plan || !plan->exec)
+    return TK_EINVAL;
+  return record_hip(hipGraphLaunch(plan->exec, (hipStream_t)stream));
+}
+
+int tk_ragged_plan_launch(tk_ragged_plan *plan, void *stream)
+{
+  if (!plan)
+    return TK_EINVAL;
+  return plan_launch(plan, (hipStream_t)stream);
+}
+
+int tk_ragged_plan_info(const tk_ragged_plan *plan, int *method, int *polys_per_wave, int *grid,
+                        size_t *npoly_total)
+{
+  if (!plan)
+    return TK_EINVAL;
+  if (method)
+    *method = plan->shape == 32 ? TK_METHOD_MFMA32 : TK_METHOD_MFMA16;
+  if (polys_per_wave)
+    *polys_per_wave = plan->shape;
+  if (grid)
+    *grid = plan->grid;
+  if (npoly_total)
+    *npoly_total = plan->npoly;
+  return TK_OK;
+}
+
+int tk_ragged_plan_destroy(tk_ragged_plan *plan)
+{
+  if (!plan)
+    return TK_EINVAL;
+  plan_free(plan);
+  return TK_OK;
+}
+
+}  // extern "C"

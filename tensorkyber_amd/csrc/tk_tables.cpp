@@ -193,7 +193,16 @@ void build_tables(HostMatrices *hm, DeviceTables *dt)
         hm->inv2[b][c][j] = (uint16_t)r[16 * b + c];
     }
 
+  // 2^-16 mod q = 169 (65536 * 169 = 3329 * 3327 + 1)
+  for (int i = 0; i < 16; i++)
+    for (int m = 0; m < 16; m++)
+      hm->inv1p[i][m] = (uint16_t)modq((int64_t)hm->inv1[i][m] * 169);
+  for (int i = 0; i < 64; i++)
+    dt->gamma[i] = (uint16_t)omega(64 + i);
+
   pack16(hm->fwd1, dt->m16_fwd1[0]);
+  pack16(hm->inv1p, dt->m16_inv1p[0]);
+  row_bias(hm->inv1p, kCentre, dt->bias_inv1p[0]);
   pack16(hm->inv1, dt->m16_inv1[0]);
   pack32(hm->fwd1, dt->m32_fwd1[0]);
   pack32(hm->inv1, dt->m32_inv1[0]);

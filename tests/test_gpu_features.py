@@ -1,0 +1,175 @@
+"""GPU parity of the ragged persistent path (hipGraph plan + autotuner) and of
+the fused NTT -> basemul -> iNTT matrix-vector product, against the oracle
+and the full-size digests of the compiled reference."""
+import ctypes
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import canon, centred
+
+pytestmark = pytest.mark.gpu
+
+N, Q = 256, 3329
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.fixture(scope="module")
+def digests():
+    with open(os.path.join(HERE, "golden", "digests.json")) as f:
+        return json.load(f)
+
+
+def _dev(a, device):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.int16)).to(device)
+
+
+def _make_plan(tk, pool_dev, reqs, direction, method):
+    table = (tk.TkRequest * len(reqs))()
+    base = pool_dev.data_ptr()
+    for r, (off, n, k, kind) in enumerate(reqs):
+        table[r].base = base + int(off) * N * 2
+        table[r].npoly, table[r].k, table[r].kind = int(n), int(k), int(kind)
+    plan = ctypes.c_void_p()
+    rc = tk.lib().tk_ragged_plan_create(ctypes.byref(plan), ctypes.addressof(table), len(reqs),
+                                        direction, method, None)
+    assert rc == 0, f"plan create failed: {rc}"
+    return plan
+
+
+def _stream():
+    import torch
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+@pytest.mark.parametrize("method", ["auto", "mfma16", "mfma32"])
+def test_ragged_plan_small(tk, gpu, oracle, method):
+    import torch
+    reqs, pool = tk.wl_ragged_layout(20000, 21)
+    x = tk.wl_uniform(pool * N, 21).reshape(pool, N)
+    d = _dev(x, gpu)
+    plan = _make_plan(tk, d, reqs, 0, tk.METHODS[method])
+    info = [ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_size_t()]
+    assert tk.lib().tk_ragged_plan_info(plan, *[ctypes.byref(v) for v in info]) == 0
+    assert info[3].value == 20000 and info[1].value in (16, 32)
+    print(f"ragged plan {method}: shape {info[1].value} grid {info[2].value}", flush=True)
+    assert tk.lib().tk_ragged_plan_execute(plan, _stream()) == 0
+    torch.cuda.synchronize()
+    got = d.cpu().numpy()
+    want = x.copy()
+    for off, n, _k, _kind in reqs:
+        want[off:off + n] = canon(oracle.ntt(x[off:off + n]))
+    assert np.array_equal(got, want), "ragged forward mismatch (or gap modified)"
+    # inverse plan on the forward output, launched without the graph
+    iplan = _make_plan(tk, d, reqs, 1, tk.METHODS[method])
+    assert tk.lib().tk_ragged_plan_launch(iplan, _stream()) == 0
+    torch.cuda.synchronize()
+    got2 = d.cpu().numpy()
+    want2 = want.copy()
+    for off, n, _k, _kind in reqs:
+        want2[off:off + n] = centred(oracle.invntt(want[off:off + n]))
+    assert np.array_equal(got2, want2)
+    assert tk.lib().tk_ragged_plan_destroy(plan) == 0
+    assert tk.lib().tk_ragged_plan_destroy(iplan) == 0
+
+
+def test_ragged_graph_replays(tk, gpu, oracle):
+    """Each replay of the captured graph matches the eager result; eager work
+    and synchronisation run between the replays."""
+    import torch
+    reqs, pool = tk.wl_ragged_layout(5000, 22)
+    x = tk.wl_uniform(pool * N, 22).reshape(pool, N)
+    d = _dev(x, gpu)
+    plan = _make_plan(tk, d, reqs, 0, tk.METHOD_MFMA16)
+    want = x.copy()
+    for off, n, _k, _kind in reqs:
+        want[off:off + n] = canon(oracle.ntt(x[off:off + n]))
+    src = _dev(x, gpu)
+    for rep in range(4):
+        d.copy_(src)
+        tk.ntt(torch.empty((64, N), dtype=torch.int16, device=gpu).zero_())  # eager work
+        torch.cuda.synchronize()
+        assert tk.lib().tk_ragged_plan_execute(plan, _stream()) == 0
+        torch.cuda.synchronize()
+        assert np.array_equal(d.cpu().numpy(), want), f"replay {rep}"
+    tk.lib().tk_ragged_plan_destroy(plan)
+
+
+def test_ragged_full_size_digest(tk, gpu, digests):
+    """Configuration 4: 2^20 polys in ragged requests, forward then inverse."""
+    import torch
+    from golden.make_golden import ragged_pool
+    reqs, pool = ragged_pool()
+    d = _dev(pool, gpu)
+    dg = digests["config4"]
+    fplan = _make_plan(tk, d, reqs, 0, tk.METHOD_AUTO)
+    iplan = _make_plan(tk, d, reqs, 1, tk.METHOD_AUTO)
+    assert tk.lib().tk_ragged_plan_execute(fplan, _stream()) == 0
+    torch.cuda.synchronize()
+    assert tk.canon_digest(d.cpu().numpy()) == dg["ntt"]
+    assert tk.lib().tk_ragged_plan_execute(iplan, _stream()) == 0
+    torch.cuda.synchronize()
+    assert tk.canon_digest(d.cpu().numpy()) == dg["invntt_of_canonical_ntt"]
+    tk.lib().tk_ragged_plan_destroy(fplan)
+    tk.lib().tk_ragged_plan_destroy(iplan)
+
+
+def test_autotune(tk, gpu):
+    m, w = ctypes.c_int(), ctypes.c_int()
+    assert tk.lib().tk_autotune(1 << 18, 0, ctypes.byref(m), ctypes.byref(w), _stream()) == 0
+    assert m.value in (tk.METHOD_MFMA16, tk.METHOD_MFMA32) and w.value >= 2
+    print(f"autotune fwd 2^18: method {m.value} waves/CU {w.value}", flush=True)
+
+
+@pytest.mark.parametrize("k,ninst", [(2, 333), (3, 1000), (4, 257), (3, 1), (3, 17)])
+def test_matvec_vs_oracle(tk, gpu, oracle, k, ninst):
+    A = tk.wl_uniform(ninst * k * k * N, 30 + k, 0, Q).reshape(ninst, k, k, N)
+    s = tk.wl_cbd2(ninst * k * N, 30 + k).reshape(ninst, k, N)
+    t = tk.matvec(_dev(A, gpu), _dev(s, gpu), k).cpu().numpy().reshape(ninst, k, N)
+    assert np.array_equal(t, centred(oracle.matvec(A, s, k)))
+
+
+def test_matvec_full_range_inputs(tk, gpu, oracle):
+    """A_hat and s over all of int16: exact mod q (the oracle runs on canonical
+    inputs, where its int16 arithmetic cannot wrap)."""
+    k, ninst = 3, 200
+    A = tk.wl_uniform(ninst * k * k * N, 40, -32768, 32768).reshape(ninst, k, k, N)
+    s = tk.wl_uniform(ninst * k * N, 41, -32768, 32768).reshape(ninst, k, N)
+    t = tk.matvec(_dev(A, gpu), _dev(s, gpu), k).cpu().numpy().reshape(ninst, k, N)
+    assert np.array_equal(t, centred(oracle.matvec(canon(A), canon(s), k)))
+
+
+def test_matvec_in_place_over_s(tk, gpu, oracle):
+    k, ninst = 3, 100
+    A = tk.wl_uniform(ninst * k * k * N, 42, 0, Q).reshape(ninst, k, k, N)
+    s = tk.wl_cbd2(ninst * k * N, 43).reshape(ninst, k, N)
+    ds = _dev(s, gpu)
+    tk.matvec(_dev(A, gpu), ds, k, out=ds)
+    assert np.array_equal(ds.cpu().numpy().reshape(ninst, k, N), centred(oracle.matvec(A, s, k)))
+
+
+def test_matvec_full_size_digest(tk, gpu, digests):
+    """Configuration 5: 65,536 instances, k = 3."""
+    from golden.make_golden import matvec_inputs
+    A, s = matvec_inputs()
+    t = tk.matvec(_dev(A, gpu), _dev(s, gpu), 3).cpu().numpy()
+    assert tk.canon_digest(t) == digests["config5"]["t"]
+
+
+@pytest.mark.parametrize("cfg", [1, 2, 3])
+@pytest.mark.parametrize("method", ["mfma16", "mfma32", "butterfly"])
+def test_full_size_digests(tk, gpu, digests, cfg, method):
+    """Configurations 1-3 at full size: forward, then inverse of the forward."""
+    import torch
+    x = tk.workload(cfg)
+    d = _dev(x, gpu)
+    tk.ntt(d, method=method)
+    torch.cuda.synchronize()
+    f = d.cpu().numpy()
+    assert tk.canon_digest(f) == digests[f"config{cfg}"]["ntt"]
+    assert f.min() >= 0 and f.max() < Q
+    tk.invntt(d, method=method)
+    assert tk.canon_digest(d.cpu().numpy()) == digests[f"config{cfg}"]["invntt_of_canonical_ntt"]
