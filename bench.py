@@ -40,33 +40,82 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-check", action="store_true", help="skip the parity check")
     ap.add_argument("--only", default=None, help="time only this method (profiling runs)")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="profiling run: launch each kernel `steps` times plus a same-size "
+                         "device copy, print per-kernel times, no CPU leg or parity check")
     return ap.parse_args()
 
 
+def copy_bandwidth(torch, src, dst, steps):
+    """A plain device copy of the batch: achievable-bandwidth and counter reference."""
+    for _ in range(2):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    return {"ms": ms, "GBs": 2 * src.numel() * 2 / (ms / 1e3) / 1e9}
+
+
 def time_method(tk, torch, method, src, buf, steps, warmup, npoly):
-    """Times `steps` round trips; returns per-kernel mean ms and wall ms/step."""
+    """Times the round trip and each kernel on the current stream.
+
+    * ``rt_ms``: `steps` forward+inverse pairs back to back between two events
+      (the whole-job step time, launch gaps included);
+    * ``fwd_ms`` / ``inv_ms``: `steps` launches of one kernel back to back
+      between two events, divided by `steps` (the per-launch average that the
+      rocprofv3 kernel-trace average is compared with);
+    * ``rt_steps_ms``: per-step times with an event between steps, for spread.
+    """
     stream = torch.cuda.current_stream()
+
+    def ev():
+        return torch.cuda.Event(enable_timing=True)
+
     for _ in range(warmup):
         tk.ntt(src, out=buf, method=method)
         tk.invntt(buf, method=method)
     torch.cuda.synchronize()
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3 * steps)]
+    e0, e1 = ev(), ev()
     t0 = time.perf_counter()
-    for s in range(steps):
-        ev[3 * s].record(stream)
+    e0.record(stream)
+    for _ in range(steps):
         tk.ntt(src, out=buf, method=method)
-        ev[3 * s + 1].record(stream)
         tk.invntt(buf, method=method)
-        ev[3 * s + 2].record(stream)
+    e1.record(stream)
     torch.cuda.synchronize()
     wall = (time.perf_counter() - t0) * 1e3 / steps
-    f = np.array([ev[3 * s].elapsed_time(ev[3 * s + 1]) for s in range(steps)])
-    i = np.array([ev[3 * s + 1].elapsed_time(ev[3 * s + 2]) for s in range(steps)])
-    rt = npoly / ((f + i) / 1e3)
+    rt_ms = e0.elapsed_time(e1) / steps
+
+    def per_kernel(fn):
+        a, b = ev(), ev()
+        a.record(stream)
+        for _ in range(steps):
+            fn()
+        b.record(stream)
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / steps
+
+    fwd_ms = per_kernel(lambda: tk.ntt(src, out=buf, method=method))
+    inv_ms = per_kernel(lambda: tk.invntt(buf, method=method))
+
+    evs = [ev() for _ in range(steps + 1)]
+    evs[0].record(stream)
+    for s in range(steps):
+        tk.ntt(src, out=buf, method=method)
+        tk.invntt(buf, method=method)
+        evs[s + 1].record(stream)
+    torch.cuda.synchronize()
+    per = np.array([evs[s].elapsed_time(evs[s + 1]) for s in range(steps)])
+    rt = npoly / (per / 1e3)
     return {
-        "fwd_ms": float(f.mean()), "inv_ms": float(i.mean()),
-        "fwd_ms_median": float(np.median(f)), "inv_ms_median": float(np.median(i)),
-        "rt_poly_s_mean": float(rt.mean()), "rt_poly_s_median": float(np.median(rt)),
+        "rt_ms": rt_ms, "fwd_ms": fwd_ms, "inv_ms": inv_ms,
+        "rt_poly_s": npoly / (rt_ms / 1e3),
+        "rt_poly_s_median": float(np.median(rt)),
         "rt_poly_s_sd": float(rt.std(ddof=1)) if steps > 1 else 0.0,
         "wall_ms_per_step": wall,
     }
@@ -137,6 +186,15 @@ def main():
     buf = torch.empty_like(src)
     torch.cuda.synchronize()
 
+    if args.kernels_only:
+        out = {"npoly": npoly}
+        for m in ([args.only] if args.only else ["mfma16", "mfma32", "butterfly"]):
+            print(f"[bench] kernels {m}", file=sys.stderr, flush=True)
+            out[m] = time_method(tk, torch, m, src, buf, args.steps, args.warmup, npoly)
+        out["copy"] = copy_bandwidth(torch, src, buf, args.steps)
+        print(json.dumps(out))
+        return
+
     methods = [args.only] if args.only else [args.method, "butterfly"]
     res = {}
     for m in methods:
@@ -145,7 +203,7 @@ def main():
 
     main_m = methods[0]
     r = res[main_m]
-    value = npoly / ((r["fwd_ms"] + r["inv_ms"]) / 1e3)
+    value = r["rt_poly_s"]
     dom = "fwd" if r["fwd_ms"] >= r["inv_ms"] else "inv"
     dom_ms = r[f"{dom}_ms"]
     achieved = BYTES_PER_POLY_PASS * npoly / (dom_ms / 1e3) / 1e9
@@ -173,7 +231,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": r["fwd_ms"] + r["inv_ms"],
+        "ms_per_step": r["rt_ms"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -200,9 +258,10 @@ def main():
         "kernels": res,
         "cpu_baseline": cpu,
     }
+    line["copy_reference"] = copy_bandwidth(torch, src, buf, max(args.steps, 5))
     if "butterfly" in res and main_m != "butterfly":
         b = res["butterfly"]
-        line["butterfly_poly_s"] = npoly / ((b["fwd_ms"] + b["inv_ms"]) / 1e3)
+        line["butterfly_poly_s"] = b["rt_poly_s"]
         line["speedup_vs_butterfly"] = value / line["butterfly_poly_s"]
     print(json.dumps(line))
 

@@ -119,14 +119,15 @@ typedef struct {
 typedef struct tk_ragged_plan tk_ragged_plan;
 int tk_ragged_plan_create(tk_ragged_plan **plan, const tk_request *table, size_t nreq,
                           int direction, int method, void *stream);
-/* Replays the captured graph on stream (NULL = the plan's stream). */
+/* Replays the captured graph on stream (NULL = the legacy default stream). */
 int tk_ragged_plan_execute(tk_ragged_plan *plan, void *stream);
 /* Launches the persistent kernel directly, without the graph. */
 int tk_ragged_plan_launch(tk_ragged_plan *plan, void *stream);
 /* Shape the plan runs: method (MFMA16 / MFMA32), polys per wave tile, grid
- * size in workgroups, and the total number of polys. */
+ * oversubscription (grid = factor x workgroups resident at once), and the
+ * total number of polys. */
 int tk_ragged_plan_info(const tk_ragged_plan *plan, int *method, int *polys_per_wave,
-                        int *grid, size_t *npoly_total);
+                        int *oversub, size_t *npoly_total);
 int tk_ragged_plan_destroy(tk_ragged_plan *plan);
 
 /* ---- fused module matrix-vector product -------------------------------- */
@@ -140,12 +141,14 @@ int matvec_ntt_fused(int16_t *t, const int16_t *A_hat, const int16_t *s, int k, 
 
 /* ---- autotuner ---------------------------------------------------------- */
 
-/* Times each MFMA shape / grid size on scratch data of npoly polys on the
- * current device and returns the fastest method (TK_METHOD_MFMA16 or
- * TK_METHOD_MFMA32) and waves per CU; results are cached per device and
- * direction for the life of the process. */
-int tk_autotune(size_t npoly, int direction, int *best_method, int *best_waves_per_cu,
-                void *stream);
+/* Times each MFMA tile shape x waves per workgroup x grid oversubscription
+ * (1, 2, 4, 8 times the workgroups resident at once) on scratch data of
+ * npoly polys (at most 2^20) on the current device and returns the fastest:
+ * method (TK_METHOD_MFMA16 or TK_METHOD_MFMA32), waves per workgroup (2 or 4)
+ * and the oversubscription factor.  Results are cached per device, direction
+ * and size octave for the life of the process.  Allocates and synchronises. */
+int tk_autotune(size_t npoly, int direction, int *best_method, int *best_waves_per_wg,
+                int *best_oversub, void *stream);
 
 /* ---- introspection (host only, no device needed) ----------------------- */
 

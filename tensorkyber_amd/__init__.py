@@ -98,7 +98,7 @@ def _declare(lib):
                                  ctypes.POINTER(sz)], i),
         "tk_ragged_plan_destroy": ([vp], i),
         "matvec_ntt_fused": ([vp, vp, vp, i, sz, vp], i),
-        "tk_autotune": ([sz, i, ctypes.POINTER(i), ctypes.POINTER(i), vp], i),
+        "tk_autotune": ([sz, i, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(i), vp], i),
     }
     for name, (args, res) in sigs.items():
         f = getattr(lib, name)

@@ -5,7 +5,9 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <map>
 #include <mutex>
+#include <utility>
 
 #include "tk_internal.h"
 #include "tk_launch.h"
@@ -140,14 +142,21 @@ int device_form(int form)
   }
 }
 
-int grid_cap(const DevState &st, int method)
+int resident_per_cu(const void *kernel, int block_threads)
 {
-  // Persistent grids: workgroups per CU that fit the LDS / VGPR budget.
-  switch (method) {
-    case TK_METHOD_MFMA16: return st.ncu * 4;
-    case TK_METHOD_MFMA32: return st.ncu * 2;
-    default: return st.ncu * 8;
-  }
+  static std::mutex mu;
+  static std::map<std::pair<const void *, int>, int> cache;
+  std::lock_guard<std::mutex> lk(mu);
+  const auto key = std::make_pair(kernel, block_threads);
+  auto it = cache.find(key);
+  if (it != cache.end())
+    return it->second;
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, block_threads, 0) != hipSuccess ||
+      n < 1)
+    n = 1;
+  cache[key] = n;
+  return n;
 }
 
 int transform(bool inverse, int16_t *out, const int16_t *in, size_t npoly, int method, int form,
@@ -168,11 +177,10 @@ int transform(bool inverse, int16_t *out, const int16_t *in, size_t npoly, int m
     return rc;
   hipStream_t s = (hipStream_t)stream;
   if (m == TK_METHOD_BUTTERFLY)
-    rc = launch_butterfly(inverse, device_form(f), out, in, npoly, st->tables,
-                          grid_cap(*st, m), s);
+    rc = launch_butterfly(inverse, device_form(f), out, in, npoly, st->tables, st->ncu, s);
   else
     rc = launch_mfma(m == TK_METHOD_MFMA32 ? 32 : 16, inverse, device_form(f), out, in, npoly,
-                     st->tables, grid_cap(*st, m), 4, s);
+                     st->tables, LaunchShape{st->ncu, 4, kDefaultOversub}, s);
   if (rc)
     return TK_ENOTSUP;
   return after_launch();
@@ -247,7 +255,7 @@ int basemul_batch_ex(int16_t *r, const int16_t *a, const int16_t *b, size_t npol
   int rc = init_device(&st);
   if (rc)
     return rc;
-  if (launch_basemul(device_form(form), r, a, b, npoly, st->tables, st->ncu * 16,
+  if (launch_basemul(device_form(form), r, a, b, npoly, st->tables, st->ncu,
                      (hipStream_t)stream))
     return TK_ENOTSUP;
   return after_launch();
@@ -277,7 +285,7 @@ int matvec_ntt_fused(int16_t *t, const int16_t *A_hat, const int16_t *s, int k, 
   int rc = init_device(&st);
   if (rc)
     return rc;
-  if (launch_matvec(k, t, A_hat, s, ninst, st->tables, st->ncu * 3, (hipStream_t)stream))
+  if (launch_matvec(k, t, A_hat, s, ninst, st->tables, st->ncu, (hipStream_t)stream))
     return TK_ENOTSUP;
   return after_launch();
 }

@@ -31,8 +31,8 @@ __global__ __launch_bounds__(256) void k_basemul(int16_t *__restrict__ r,
       v4i v;
       int16_t h[8];
     } va, vb, vr;
-    va.v = reinterpret_cast<const v4i *>(a)[ch];
-    vb.v = reinterpret_cast<const v4i *>(b)[ch];
+    va.v = ld_stream(reinterpret_cast<const v4i *>(a) + ch);
+    vb.v = ld_stream(reinterpret_cast<const v4i *>(b) + ch);
     const int i0 = (int)(ch & 31) * 2;  // group of 4 coefficients within the poly
 #pragma unroll
     for (int q = 0; q < 2; q++) {
@@ -51,22 +51,27 @@ __global__ __launch_bounds__(256) void k_basemul(int16_t *__restrict__ r,
       for (int j = 0; j < 8; j++)
         vr.h[j] = centre16(vr.h[j]);
     }
-    reinterpret_cast<v4i *>(r)[ch] = vr.v;
+    st_stream(reinterpret_cast<v4i *>(r) + ch, vr.v);
   }
 }
 
 }  // namespace
 
 int launch_basemul(int form, int16_t *r, const int16_t *a, const int16_t *b, size_t npoly,
-                   const DeviceTables *tab, int grid_cap, hipStream_t stream)
+                   const DeviceTables *tab, int ncu, hipStream_t stream)
 {
   const size_t nchunk = npoly * (kN / 8);
   const size_t need = (nchunk + 255) / 256;
-  const dim3 grid((unsigned)(need < (size_t)grid_cap ? need : (size_t)grid_cap));
+  auto go = [&](auto kernel) {
+    const size_t cap = (size_t)ncu * resident_per_cu((const void *)kernel, 256);
+    const dim3 grid((unsigned)(need < cap ? need : cap));
+    hipLaunchKernelGGL(kernel, grid, dim3(256), 0, stream, r, a, b, npoly, tab);
+    return 0;
+  };
   switch (form) {
-    case kFormRef: hipLaunchKernelGGL(k_basemul<kFormRef>, grid, dim3(256), 0, stream, r, a, b, npoly, tab); return 0;
-    case kFormCanon: hipLaunchKernelGGL(k_basemul<kFormCanon>, grid, dim3(256), 0, stream, r, a, b, npoly, tab); return 0;
-    case kFormCentred: hipLaunchKernelGGL(k_basemul<kFormCentred>, grid, dim3(256), 0, stream, r, a, b, npoly, tab); return 0;
+    case kFormRef: return go(k_basemul<kFormRef>);
+    case kFormCanon: return go(k_basemul<kFormCanon>);
+    case kFormCentred: return go(k_basemul<kFormCentred>);
     default: return -1;
   }
 }

@@ -54,7 +54,7 @@ __device__ __forceinline__ void stage_in(int16_t *L, const int16_t *__restrict__
     const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
     v4i v = {0, 0, 0, 0};
     if (p0 + pp < npoly)
-      v = reinterpret_cast<const v4i *>(in + (p0 + pp) * kN)[ch];
+      v = ld_stream(reinterpret_cast<const v4i *>(in + (p0 + pp) * kN) + ch);
     *reinterpret_cast<v4i *>(L + pp * kStride + ch * 8) = v;
   }
 }
@@ -67,7 +67,7 @@ __device__ __forceinline__ void stage_out(const int16_t *L, int16_t *__restrict_
     const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
     const v4i v = *reinterpret_cast<const v4i *>(L + pp * kStride + ch * 8);
     if (p0 + pp < npoly)
-      reinterpret_cast<v4i *>(out + (p0 + pp) * kN)[ch] = v;
+      st_stream(reinterpret_cast<v4i *>(out + (p0 + pp) * kN) + ch, v);
   }
 }
 
@@ -239,27 +239,23 @@ __global__ __launch_bounds__(256) void k_invntt_butterfly(int16_t *__restrict__ 
 }  // namespace
 
 int launch_butterfly(bool inverse, int form, int16_t *out, const int16_t *in, size_t npoly,
-                     const DeviceTables *tab, int grid, hipStream_t stream)
+                     const DeviceTables *tab, int ncu, hipStream_t stream)
 {
   const size_t ngroups = (npoly + kPolysPerWave - 1) / kPolysPerWave;
   const size_t need = (ngroups + kWaves - 1) / kWaves;
-  const unsigned g = (unsigned)(need < (size_t)grid ? need : (size_t)grid);
-  const dim3 blk(64 * kWaves);
-#define TK_BF_LAUNCH(F)                                                                   \
-  do {                                                                                    \
-    if (inverse)                                                                          \
-      hipLaunchKernelGGL(k_invntt_butterfly<F>, dim3(g), blk, 0, stream, out, in, npoly, tab); \
-    else                                                                                  \
-      hipLaunchKernelGGL(k_ntt_butterfly<F>, dim3(g), blk, 0, stream, out, in, npoly, tab);    \
-  } while (0)
+  const int block = 64 * kWaves;
+  auto go = [&](auto kernel) {
+    const size_t cap = (size_t)ncu * resident_per_cu((const void *)kernel, block);
+    const dim3 grid((unsigned)(need < cap ? need : cap));
+    hipLaunchKernelGGL(kernel, grid, dim3(block), 0, stream, out, in, npoly, tab);
+    return 0;
+  };
   switch (form) {
-    case kFormCanon: TK_BF_LAUNCH(kFormCanon); break;
-    case kFormCentred: TK_BF_LAUNCH(kFormCentred); break;
-    case kFormRef: TK_BF_LAUNCH(kFormRef); break;
+    case kFormCanon: return inverse ? go(k_invntt_butterfly<kFormCanon>) : go(k_ntt_butterfly<kFormCanon>);
+    case kFormCentred: return inverse ? go(k_invntt_butterfly<kFormCentred>) : go(k_ntt_butterfly<kFormCentred>);
+    case kFormRef: return inverse ? go(k_invntt_butterfly<kFormRef>) : go(k_ntt_butterfly<kFormRef>);
     default: return -1;
   }
-#undef TK_BF_LAUNCH
-  return 0;
 }
 
 }  // namespace tk

@@ -97,7 +97,7 @@ __global__ __launch_bounds__(64) void k_matvec(int16_t *__restrict__ t,
               v4i v;
               int16_t h[8];
             } a, b;
-            a.v = reinterpret_cast<const v4i *>(A + (((i0 + pp) * K + i) * K + j) * kN)[ch];
+            a.v = ld_stream(reinterpret_cast<const v4i *>(A + (((i0 + pp) * K + i) * K + j) * kN) + ch);
             b.v = *reinterpret_cast<const v4i *>(s_hat[j] + pp * kStride + ch * 8);
 #pragma unroll
             for (int u = 0; u < 4; u++) {
@@ -138,14 +138,19 @@ __global__ __launch_bounds__(64) void k_matvec(int16_t *__restrict__ t,
 }  // namespace
 
 int launch_matvec(int k, int16_t *t, const int16_t *A, const int16_t *s, size_t ninst,
-                  const DeviceTables *tab, int grid_cap, hipStream_t stream)
+                  const DeviceTables *tab, int ncu, hipStream_t stream)
 {
   const size_t ntiles = (ninst + 15) / 16;
-  const dim3 grid((unsigned)(ntiles < (size_t)grid_cap ? ntiles : (size_t)grid_cap));
+  auto go = [&](auto kernel) {
+    const size_t cap = (size_t)ncu * resident_per_cu((const void *)kernel, 64);
+    const dim3 grid((unsigned)(ntiles < cap ? ntiles : cap));
+    hipLaunchKernelGGL(kernel, grid, dim3(64), 0, stream, t, A, s, ninst, tab);
+    return 0;
+  };
   switch (k) {
-    case 2: hipLaunchKernelGGL(k_matvec<2>, grid, dim3(64), 0, stream, t, A, s, ninst, tab); return 0;
-    case 3: hipLaunchKernelGGL(k_matvec<3>, grid, dim3(64), 0, stream, t, A, s, ninst, tab); return 0;
-    case 4: hipLaunchKernelGGL(k_matvec<4>, grid, dim3(64), 0, stream, t, A, s, ninst, tab); return 0;
+    case 2: return go(k_matvec<2>);
+    case 3: return go(k_matvec<3>);
+    case 4: return go(k_matvec<4>);
     default: return -1;
   }
 }

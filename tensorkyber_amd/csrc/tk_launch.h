@@ -26,19 +26,34 @@ struct RaggedTiles {
   size_t npoly;
 };
 
+// Grid shape of the tile kernels: waves per workgroup, and the grid as a
+// multiple of the workgroups resident at once (from the occupancy
+// calculator).  Each wave strides over tiles; a factor above 1 lets the
+// dispatcher hand late workgroups to whichever CU frees up first, which
+// balances the load (4 measured best for the 16-poly tiles).
+constexpr int kDefaultOversub = 4;
+struct LaunchShape {
+  int ncu;
+  int waves;
+  int oversub;
+};
+
+// Workgroups of `block_threads` that fit on one CU at once (cached).
+int resident_per_cu(const void *kernel, int block_threads);
+
 int launch_butterfly(bool inverse, int form, int16_t *out, const int16_t *in, size_t npoly,
-                     const DeviceTables *tab, int grid_cap, hipStream_t stream);
+                     const DeviceTables *tab, int ncu, hipStream_t stream);
 
 int launch_mfma(int shape, bool inverse, int form, int16_t *out, const int16_t *in, size_t npoly,
-                const DeviceTables *tab, int grid_cap, int waves, hipStream_t stream);
+                const DeviceTables *tab, const LaunchShape &ls, hipStream_t stream);
 
 int launch_mfma_ragged(int shape, bool inverse, int form, const RaggedTiles &io,
-                       const DeviceTables *tab, int grid_cap, int waves, hipStream_t stream);
+                       const DeviceTables *tab, const LaunchShape &ls, hipStream_t stream);
 
 int launch_matvec(int k, int16_t *t, const int16_t *A, const int16_t *s, size_t ninst,
-                  const DeviceTables *tab, int grid_cap, hipStream_t stream);
+                  const DeviceTables *tab, int ncu, hipStream_t stream);
 
 int launch_basemul(int form, int16_t *r, const int16_t *a, const int16_t *b, size_t npoly,
-                   const DeviceTables *tab, int grid_cap, hipStream_t stream);
+                   const DeviceTables *tab, int ncu, hipStream_t stream);
 
 }  // namespace tk

@@ -34,43 +34,40 @@ __device__ __forceinline__ int32_t bias_adjust()
   return FORM == kFormCentred ? 0 : -kCentre;
 }
 
-// Per-tile poly addresses.  Linear batches compute them; ragged batches walk
-// the request table once per tile (lanes 0..P-1, one poly each) and park the
-// pointers in LDS for the staging loops.
+// Per-tile poly addresses.  Lanes 0..P-1 compute one poly address each
+// (linear: from the tile index; ragged: by walking the request table) and
+// park them in LDS, where the staging loops read them back; keeping them out
+// of vector registers across the two passes saves about 40 VGPRs, which is
+// one more resident wave per SIMD.
 template <int P>
-struct TileAddr {
-  const int16_t *in0;
-  int16_t *out0;
-  int16_t **lds;  // ragged only
-  size_t valid;   // linear only: polys of this tile inside the batch
-  __device__ const int16_t *src(int pp) const
-  {
-    if (lds)
-      return lds[pp];
-    return (size_t)pp < valid ? in0 + (size_t)pp * kN : nullptr;
-  }
-  __device__ int16_t *dst(int pp) const
-  {
-    if (lds)
-      return lds[pp];
-    return (size_t)pp < valid ? out0 + (size_t)pp * kN : nullptr;
-  }
-};
-
-template <int P>
-__device__ __forceinline__ TileAddr<P> tile_addr(const LinearTiles &io, size_t tile, int lane,
-                                                 int16_t **lds)
+__device__ __forceinline__ void tile_prepare(const LinearTiles &io, size_t tile, int lane,
+                                             int16_t **lds)
 {
-  (void)lane;
-  (void)lds;
-  const size_t p0 = tile * P;
-  const size_t left = io.npoly - p0;
-  return TileAddr<P>{io.in + p0 * kN, io.out + p0 * kN, nullptr, left < (size_t)P ? left : P};
+  if (lane < P) {
+    const size_t p = tile * P + lane;
+    lds[lane] = p < io.npoly ? const_cast<int16_t *>(io.in) + p * kN : nullptr;
+    lds[P + lane] = p < io.npoly ? io.out + p * kN : nullptr;
+  }
+  wave_sync();
 }
 
 template <int P>
-__device__ __forceinline__ TileAddr<P> tile_addr(const RaggedTiles &io, size_t tile, int lane,
-                                                 int16_t **lds)
+__device__ __forceinline__ const int16_t *tile_src(const LinearTiles &, size_t, int pp,
+                                                   int16_t *const *lds)
+{
+  return lds[pp];
+}
+
+template <int P>
+__device__ __forceinline__ int16_t *tile_dst(const LinearTiles &, size_t, int pp,
+                                             int16_t *const *lds)
+{
+  return lds[P + pp];
+}
+
+template <int P>
+__device__ __forceinline__ void tile_prepare(const RaggedTiles &io, size_t tile, int lane,
+                                             int16_t **lds)
 {
   if (lane < P) {
     const uint64_t g = (uint64_t)tile * P + lane;
@@ -84,7 +81,20 @@ __device__ __forceinline__ TileAddr<P> tile_addr(const RaggedTiles &io, size_t t
     lds[lane] = p;
   }
   wave_sync();
-  return TileAddr<P>{nullptr, nullptr, lds, 0};
+}
+
+template <int P>
+__device__ __forceinline__ const int16_t *tile_src(const RaggedTiles &, size_t, int pp,
+                                                   int16_t *const *lds)
+{
+  return lds[pp];
+}
+
+template <int P>
+__device__ __forceinline__ int16_t *tile_dst(const RaggedTiles &, size_t, int pp,
+                                             int16_t *const *lds)
+{
+  return lds[pp];
 }
 
 template <int FORM, bool INV, int WAVES, class Tiles>
@@ -93,7 +103,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_mfma16(Tiles io,
 {
   constexpr int P = 16;
   __shared__ __align__(16) int16_t s_poly[WAVES][P * kStride];
-  __shared__ int16_t *s_ptr[WAVES][P];
+  __shared__ int16_t *s_ptr[WAVES][2 * P];
   __shared__ __align__(16) int32_t s_bias[16][16];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, g = lane >> 4, t = lane & 15;
 
@@ -119,11 +129,16 @@ __global__ __launch_bounds__(64 * WAVES) void k_mfma16(Tiles io,
 
   int16_t *L = s_poly[w];
   int16_t *Pp = L + t * kStride;
+  // Grid-stride over tiles.  Latency is hidden by the other resident waves
+  // (three per SIMD at this register count) and by launching several times
+  // more workgroups than are resident, which also balances load across CUs;
+  // a register prefetch of the next tile measured slower (it costs a wave of
+  // occupancy).
   const size_t ntiles = (io.npoly + P - 1) / P;
   for (size_t tile = (size_t)blockIdx.x * WAVES + w; tile < ntiles;
        tile += (size_t)gridDim.x * WAVES) {
-    const TileAddr<P> ta = tile_addr<P>(io, tile, lane, s_ptr[w]);
-    tile_in<P>(L, [&](int pp) { return ta.src(pp); }, lane);
+    tile_prepare<P>(io, tile, lane, s_ptr[w]);
+    tile_in<P>(L, [&](int pp) { return tile_src<P>(io, tile, pp, s_ptr[w]); }, lane);
     wave_sync();
     if (!INV) {
       pass_m16<kFormInter>(Pp, g, a1lo, a1hi, bias1);
@@ -137,7 +152,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_mfma16(Tiles io,
       pass_m16<FORM>(Pp, g, a1lo, a1hi, bias1);
     }
     wave_sync();
-    tile_out<P>(L, [&](int pp) { return ta.dst(pp); }, lane);
+    tile_out<P>(L, [&](int pp) { return tile_dst<P>(io, tile, pp, s_ptr[w]); }, lane);
     wave_sync();
   }
 }
@@ -148,7 +163,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_mfma32(Tiles io,
 {
   constexpr int P = 32;
   __shared__ __align__(16) int16_t s_poly[WAVES][P * kStride];
-  __shared__ int16_t *s_ptr[WAVES][P];
+  __shared__ int16_t *s_ptr[WAVES][2 * P];
   __shared__ __align__(16) int32_t s_bias[16][16];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, h = lane >> 5, t = lane & 31;
 
@@ -173,11 +188,12 @@ __global__ __launch_bounds__(64 * WAVES) void k_mfma32(Tiles io,
 
   int16_t *L = s_poly[w];
   int16_t *Pp = L + t * kStride;
+  // 32 polys per wave: no register budget left for a prefetch buffer.
   const size_t ntiles = (io.npoly + P - 1) / P;
   for (size_t tile = (size_t)blockIdx.x * WAVES + w; tile < ntiles;
        tile += (size_t)gridDim.x * WAVES) {
-    const TileAddr<P> ta = tile_addr<P>(io, tile, lane, s_ptr[w]);
-    tile_in<P>(L, [&](int pp) { return ta.src(pp); }, lane);
+    tile_prepare<P>(io, tile, lane, s_ptr[w]);
+    tile_in<P>(L, [&](int pp) { return tile_src<P>(io, tile, pp, s_ptr[w]); }, lane);
     wave_sync();
     if (!INV) {
       pass_m32<kFormInter>(Pp, h, a1, bias1a, bias1b);
@@ -189,68 +205,73 @@ __global__ __launch_bounds__(64 * WAVES) void k_mfma32(Tiles io,
       pass_m32<FORM>(Pp, h, a1, bias1a, bias1b);
     }
     wave_sync();
-    tile_out<P>(L, [&](int pp) { return ta.dst(pp); }, lane);
+    tile_out<P>(L, [&](int pp) { return tile_dst<P>(io, tile, pp, s_ptr[w]); }, lane);
     wave_sync();
   }
 }
 
-template <int SHAPE, int FORM, bool INV, class Tiles>
-void launch_one(dim3 grid, int waves, const Tiles &io, const DeviceTables *tab, hipStream_t s)
+template <class K, class Tiles>
+int launch_persistent(K kernel, int waves, int P, int ncu, int oversub, const Tiles &io,
+                      const DeviceTables *tab, hipStream_t s)
 {
-  if (SHAPE == 16) {
-    if (waves == 2)
-      hipLaunchKernelGGL((k_mfma16<FORM, INV, 2, Tiles>), grid, dim3(128), 0, s, io, tab);
-    else
-      hipLaunchKernelGGL((k_mfma16<FORM, INV, 4, Tiles>), grid, dim3(256), 0, s, io, tab);
-  } else {
-    if (waves == 2)
-      hipLaunchKernelGGL((k_mfma32<FORM, INV, 2, Tiles>), grid, dim3(128), 0, s, io, tab);
-    else
-      hipLaunchKernelGGL((k_mfma32<FORM, INV, 4, Tiles>), grid, dim3(256), 0, s, io, tab);
-  }
+  const int block = 64 * waves;
+  const int per_cu = resident_per_cu((const void *)kernel, block) * (oversub > 0 ? oversub : 1);
+  const size_t ntiles = (io.npoly + P - 1) / P;
+  const size_t need = (ntiles + waves - 1) / waves;
+  const size_t cap = (size_t)ncu * per_cu;
+  const dim3 grid((unsigned)(need < cap ? need : cap));
+  hipLaunchKernelGGL(kernel, grid, dim3(block), 0, s, io, tab);
+  return 0;
+}
+
+template <int SHAPE, int FORM, bool INV, class Tiles>
+int launch_one(int waves, int ncu, int wgs, const Tiles &io, const DeviceTables *tab,
+               hipStream_t s)
+{
+  if (SHAPE == 16)
+    return waves == 2 ? launch_persistent(k_mfma16<FORM, INV, 2, Tiles>, 2, 16, ncu, wgs, io, tab, s)
+                      : launch_persistent(k_mfma16<FORM, INV, 4, Tiles>, 4, 16, ncu, wgs, io, tab, s);
+  return waves == 2 ? launch_persistent(k_mfma32<FORM, INV, 2, Tiles>, 2, 32, ncu, wgs, io, tab, s)
+                    : launch_persistent(k_mfma32<FORM, INV, 4, Tiles>, 4, 32, ncu, wgs, io, tab, s);
 }
 
 template <int SHAPE, bool INV, class Tiles>
-int launch_form(int form, dim3 grid, int waves, const Tiles &io, const DeviceTables *tab,
+int launch_form(int form, int waves, int ncu, int wgs, const Tiles &io, const DeviceTables *tab,
                 hipStream_t s)
 {
   switch (form) {
-    case kFormCanon: launch_one<SHAPE, kFormCanon, INV>(grid, waves, io, tab, s); return 0;
-    case kFormCentred: launch_one<SHAPE, kFormCentred, INV>(grid, waves, io, tab, s); return 0;
+    case kFormCanon: return launch_one<SHAPE, kFormCanon, INV>(waves, ncu, wgs, io, tab, s);
+    case kFormCentred: return launch_one<SHAPE, kFormCentred, INV>(waves, ncu, wgs, io, tab, s);
     default: return -1;
   }
 }
 
 template <class Tiles>
 int launch_any(int shape, bool inverse, int form, const Tiles &io, const DeviceTables *tab,
-               int grid_cap, int waves, hipStream_t stream)
+               const LaunchShape &ls, hipStream_t stream)
 {
-  if (waves != 2 && waves != 4)
+  if (ls.waves != 2 && ls.waves != 4)
     return -1;
-  const int P = shape == 32 ? 32 : 16;
-  const size_t ntiles = (io.npoly + P - 1) / P;
-  const size_t need = (ntiles + waves - 1) / waves;
-  const dim3 grid((unsigned)(need < (size_t)grid_cap ? need : (size_t)grid_cap));
   if (shape == 32)
-    return inverse ? launch_form<32, true>(form, grid, waves, io, tab, stream)
-                   : launch_form<32, false>(form, grid, waves, io, tab, stream);
-  return inverse ? launch_form<16, true>(form, grid, waves, io, tab, stream)
-                 : launch_form<16, false>(form, grid, waves, io, tab, stream);
+    return inverse ? launch_form<32, true>(form, ls.waves, ls.ncu, ls.oversub, io, tab, stream)
+                   : launch_form<32, false>(form, ls.waves, ls.ncu, ls.oversub, io, tab, stream);
+  return inverse ? launch_form<16, true>(form, ls.waves, ls.ncu, ls.oversub, io, tab, stream)
+                 : launch_form<16, false>(form, ls.waves, ls.ncu, ls.oversub, io, tab, stream);
 }
 
 }  // namespace
 
 int launch_mfma(int shape, bool inverse, int form, int16_t *out, const int16_t *in, size_t npoly,
-                const DeviceTables *tab, int grid_cap, int waves, hipStream_t stream)
+                const DeviceTables *tab, const LaunchShape &ls, hipStream_t stream)
 {
   const LinearTiles io{in, out, npoly};
-  return launch_any(shape, inverse, form, io, tab, grid_cap, waves, stream);
+  return launch_any(shape, inverse, form, io, tab, ls, stream);
 }
 
 int launch_mfma_ragged(int shape, bool inverse, int form, const RaggedTiles &io,
-                       const DeviceTables *tab, int grid_cap, int waves, hipStream_t stream)
+                       const DeviceTables *tab, const LaunchShape &ls, hipStream_t stream)
 {
-  return launch_any(shape, inverse, form, io, tab, grid_cap, waves, stream);
+  return launch_any(shape, inverse, form, io, tab, ls, stream);
 }
 
 }  // namespace tk

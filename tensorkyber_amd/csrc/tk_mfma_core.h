@@ -9,6 +9,10 @@
 namespace tk {
 
 constexpr int kStride = 264;  // int16 per poly slot in LDS (512 + 16 bytes)
+#ifndef TK_PASS_GROUP
+#define TK_PASS_GROUP 4
+#endif
+constexpr int kPassGroup = TK_PASS_GROUP;  // MFMA pairs in flight per pass step
 
 // ---------------------------------------------------------------------------
 // Tile staging: P polys, coalesced 16-byte global accesses, one LDS slab.
@@ -22,8 +26,31 @@ __device__ __forceinline__ void tile_in(int16_t *L, Addr src, int lane)
   for (int it = 0; it < P / 2; it++) {
     const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
     const int16_t *p = src(pp);
-    v[it] = p ? reinterpret_cast<const v4i *>(p)[ch] : v4i{0, 0, 0, 0};
+    v[it] = p ? ld_stream(reinterpret_cast<const v4i *>(p) + ch) : v4i{0, 0, 0, 0};
   }
+#pragma unroll
+  for (int it = 0; it < P / 2; it++) {
+    const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
+    *reinterpret_cast<v4i *>(L + pp * kStride + ch * 8) = v[it];
+  }
+}
+
+// Split form of tile_in for software pipelining: tile_load issues the global
+// loads of a tile into registers, tile_park writes them to the LDS slab.
+template <int P, class Addr>
+__device__ __forceinline__ void tile_load(v4i *v, Addr src, int lane)
+{
+#pragma unroll
+  for (int it = 0; it < P / 2; it++) {
+    const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
+    const int16_t *p = src(pp);
+    v[it] = p ? ld_stream(reinterpret_cast<const v4i *>(p) + ch) : v4i{0, 0, 0, 0};
+  }
+}
+
+template <int P>
+__device__ __forceinline__ void tile_park(int16_t *L, const v4i *v, int lane)
+{
 #pragma unroll
   for (int it = 0; it < P / 2; it++) {
     const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
@@ -40,7 +67,7 @@ __device__ __forceinline__ void tile_out(const int16_t *L, Addr dst, int lane)
     const v4i v = *reinterpret_cast<const v4i *>(L + pp * kStride + ch * 8);
     int16_t *p = dst(pp);
     if (p)
-      reinterpret_cast<v4i *>(p)[ch] = v;
+      st_stream(reinterpret_cast<v4i *>(p) + ch, v);
   }
 }
 
@@ -100,10 +127,12 @@ __device__ __forceinline__ void st2(int16_t *p, uint32_t a, uint32_t b)
 // before their results are read.
 // ---------------------------------------------------------------------------
 
-// Contraction over m for each column c (one matrix for all columns).
-template <int MODE>
+// Contraction over m for each column c (one matrix for all columns).  G
+// columns (2 or 4) per group: G pairs of MFMAs are in flight at once.
+template <int MODE, int G = kPassGroup>
 __device__ __forceinline__ void pass_m16(int16_t *P, int g, long alo, long ahi, v4i bias)
 {
+  static_assert(G == 2 || G == 4, "group of 2 or 4 columns");
   const v4i zero = {0, 0, 0, 0};
   int16_t *rows = P + 64 * g;  // row 4g; row 4g + r at +16r
 #pragma unroll
@@ -114,49 +143,57 @@ __device__ __forceinline__ void pass_m16(int16_t *P, int g, long alo, long ahi, 
       R[r] = ld2(rows + 16 * r + 4 * cq);
     long b[4];
     col_digits4(R, b);
-    v4i lo[4], hi[4];
 #pragma unroll
-    for (int u = 0; u < 4; u++)
-      lo[u] = mfma16(alo, b[u], bias);
+    for (int h = 0; h < 4; h += G) {
+      v4i lo[G], hi[G];
 #pragma unroll
-    for (int u = 0; u < 4; u++)
-      hi[u] = mfma16(ahi, b[u], zero);
+      for (int u = 0; u < G; u++)
+        lo[u] = mfma16(alo, b[h + u], bias);
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-      uint32_t v[4];
+      for (int u = 0; u < G; u++)
+        hi[u] = mfma16(ahi, b[h + u], zero);
 #pragma unroll
-      for (int u = 0; u < 4; u++)
-        v[u] = reduce_2q(lo[u][r], hi[u][r]);
-      st2(rows + 16 * r + 4 * cq, finish2<MODE>(pack2(v[0], v[1])),
-          finish2<MODE>(pack2(v[2], v[3])));
+      for (int r = 0; r < 4; r++) {
+        uint32_t v[G];
+#pragma unroll
+        for (int u = 0; u < G; u++)
+          v[u] = reduce_2q(lo[u][r], hi[u][r]);
+        if (G == 4)
+          st2(rows + 16 * r + 4 * cq, finish2<MODE>(pack2(v[0], v[1])),
+              finish2<MODE>(pack2(v[G - 2], v[G - 1])));
+        else
+          *reinterpret_cast<uint32_t *>(rows + 16 * r + 4 * cq + h) =
+              finish2<MODE>(pack2(v[0], v[G - 1]));
+      }
     }
   }
 }
 
-// Contraction over c inside each block m (one matrix per block).
-template <int MODE, class ALo, class AHi>
+// Contraction over c inside each block m (one matrix per block), G blocks
+// per group.
+template <int MODE, class ALo, class AHi, int G = kPassGroup>
 __device__ __forceinline__ void pass_c16(int16_t *P, int g, ALo alo, AHi ahi,
                                          const int32_t (*s_bias)[16])
 {
   const v4i zero = {0, 0, 0, 0};
 #pragma unroll
-  for (int m0 = 0; m0 < 16; m0 += 4) {
-    long b[4];
-    v4i bias[4], lo[4], hi[4];
+  for (int m0 = 0; m0 < 16; m0 += G) {
+    long b[G];
+    v4i bias[G], lo[G], hi[G];
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
+    for (int u = 0; u < G; u++) {
       const uint2 d = ld2(P + 16 * (m0 + u) + 4 * g);
       b[u] = digits8(d.x, d.y);
       bias[u] = *reinterpret_cast<const v4i *>(&s_bias[m0 + u][4 * g]);
     }
 #pragma unroll
-    for (int u = 0; u < 4; u++)
+    for (int u = 0; u < G; u++)
       lo[u] = mfma16(alo(m0 + u), b[u], bias[u]);
 #pragma unroll
-    for (int u = 0; u < 4; u++)
+    for (int u = 0; u < G; u++)
       hi[u] = mfma16(ahi(m0 + u), b[u], zero);
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
+    for (int u = 0; u < G; u++) {
       uint32_t r[4];
 #pragma unroll
       for (int i = 0; i < 4; i++)

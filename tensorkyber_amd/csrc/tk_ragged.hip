@@ -23,7 +23,8 @@ struct tk_ragged_plan {
   int direction = 0;
   int shape = 16;
   int waves = 4;
-  int grid = 0;
+  int ncu = 0;
+  int oversub = 0;
   size_t npoly = 0;
   size_t nreq = 0;
   void *dmem = nullptr;
@@ -44,7 +45,7 @@ struct TuneKey {
   }
 };
 struct TuneVal {
-  int method, waves, wgs_per_cu;
+  int method, waves, oversub;
 };
 std::mutex g_tune_mu;
 std::map<TuneKey, TuneVal> g_tune;
@@ -55,14 +56,6 @@ int bucket_of(size_t npoly)
   while (b < 40 && ((size_t)1 << (b + 1)) <= npoly)
     b++;
   return b;
-}
-
-// Workgroups per CU that fit in 160 KiB of LDS for a shape / waves choice.
-int max_wgs_per_cu(int shape, int waves)
-{
-  const int lds = waves * shape * 528 + 1024 + waves * shape * 8;
-  int n = (160 * 1024) / lds;
-  return n < 1 ? 1 : (n > 8 ? 8 : n);
 }
 
 int time_config(DevState *st, int shape, int waves, int wgs, bool inverse, int16_t *buf,
@@ -76,12 +69,12 @@ int time_config(DevState *st, int shape, int waves, int wgs, bool inverse, int16
     return TK_EDEVICE;
   }
   const int form = inverse ? kFormCentred : kFormCanon;
-  const int grid = st->ncu * wgs;
+  const LaunchShape ls{st->ncu, waves, wgs};
   float best = 1e30f;
   int rc = TK_OK;
   for (int rep = 0; rep < 4 && rc == TK_OK; rep++) {
     (void)hipEventRecord(e0, s);
-    launch_mfma(shape, inverse, form, buf, buf, npoly, st->tables, grid, waves, s);
+    launch_mfma(shape, inverse, form, buf, buf, npoly, st->tables, ls, s);
     (void)hipEventRecord(e1, s);
     if ((rc = record_hip(hipEventSynchronize(e1))) != TK_OK)
       break;
@@ -114,7 +107,9 @@ int autotune(DevState *st, size_t npoly, int direction, TuneVal *out, hipStream_
   if (hipMalloc(&buf, n * kN * sizeof(int16_t)) != hipSuccess)
     return TK_ENOMEM;
   int rc = record_hip(hipMemsetAsync(buf, 0, n * kN * sizeof(int16_t), s));
-  TuneVal best{TK_METHOD_MFMA16, 4, 4};
+  // Candidates: tile shape x waves per workgroup x grid oversubscription
+  // (grid = factor x workgroups resident at once).
+  TuneVal best{TK_METHOD_MFMA16, 4, kDefaultOversub};
   float best_ms = 1e30f;
   const int shapes[2] = {16, 32};
   const int wavess[2] = {2, 4};
@@ -123,8 +118,6 @@ int autotune(DevState *st, size_t npoly, int direction, TuneVal *out, hipStream_
     for (int wi = 0; wi < 2 && rc == TK_OK; wi++)
       for (int gi = 0; gi < 4 && rc == TK_OK; gi++) {
         const int shape = shapes[si], waves = wavess[wi], wgs = wgss[gi];
-        if (wgs > max_wgs_per_cu(shape, waves))
-          continue;
         float ms = 0;
         rc = time_config(st, shape, waves, wgs, direction != 0, buf, n, s, &ms);
         if (rc == TK_OK && ms < best_ms) {
@@ -156,8 +149,8 @@ void plan_free(tk_ragged_plan *p)
 int plan_launch(tk_ragged_plan *p, hipStream_t s)
 {
   const int form = p->direction ? kFormCentred : kFormCanon;
-  if (launch_mfma_ragged(p->shape, p->direction != 0, form, p->tiles, p->tab, p->grid, p->waves,
-                         s))
+  if (launch_mfma_ragged(p->shape, p->direction != 0, form, p->tiles, p->tab,
+                         LaunchShape{p->ncu, p->waves, p->oversub}, s))
     return TK_ENOTSUP;
   return after_launch();
 }
@@ -169,8 +162,8 @@ using namespace tk;
 
 extern "C" {
 
-int tk_autotune(size_t npoly, int direction, int *best_method, int *best_waves_per_cu,
-                void *stream)
+int tk_autotune(size_t npoly, int direction, int *best_method, int *best_waves_per_wg,
+                int *best_oversub, void *stream)
 {
   if (npoly == 0 || (direction != 0 && direction != 1) || npoly > kMaxPolys)
     return TK_EINVAL;
@@ -183,8 +176,10 @@ int tk_autotune(size_t npoly, int direction, int *best_method, int *best_waves_p
     return rc;
   if (best_method)
     *best_method = v.method;
-  if (best_waves_per_cu)
-    *best_waves_per_cu = v.waves * v.wgs_per_cu;
+  if (best_waves_per_wg)
+    *best_waves_per_wg = v.waves;
+  if (best_oversub)
+    *best_oversub = v.oversub;
   return TK_OK;
 }
 
@@ -216,14 +211,14 @@ int tk_ragged_plan_create(tk_ragged_plan **plan, const tk_request *table, size_t
     return rc;
   hipStream_t s = (hipStream_t)stream;
 
-  TuneVal tv{TK_METHOD_MFMA16, 4, 4};
+  TuneVal tv{TK_METHOD_MFMA16, 4, kDefaultOversub};
   if (method == TK_METHOD_AUTO) {
     if ((rc = autotune(st, total, direction, &tv, s)))
       return rc;
   } else {
     tv.method = method == TK_METHOD_MFMA32 ? TK_METHOD_MFMA32 : TK_METHOD_MFMA16;
     tv.waves = 4;
-    tv.wgs_per_cu = tv.method == TK_METHOD_MFMA32 ? 2 : 4;
+    tv.oversub = kDefaultOversub;
   }
   const int shape = tv.method == TK_METHOD_MFMA32 ? 32 : 16;
 
@@ -257,7 +252,8 @@ int tk_ragged_plan_create(tk_ragged_plan **plan, const tk_request *table, size_t
   p->direction = direction;
   p->shape = shape;
   p->waves = tv.waves;
-  p->grid = st->ncu * tv.wgs_per_cu;
+  p->ncu = st->ncu;
+  p->oversub = tv.oversub;
   p->npoly = total;
   p->nreq = nreq;
   p->tab = st->tables;
@@ -322,7 +318,7 @@ int tk_ragged_plan_info(const tk_ragged_plan *plan, int *method, int *polys_per_
   if (polys_per_wave)
     *polys_per_wave = plan->shape;
   if (grid)
-    *grid = plan->grid;
+    *grid = plan->oversub;
   if (npoly_total)
     *npoly_total = plan->npoly;
   return TK_OK;

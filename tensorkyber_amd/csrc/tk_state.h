@@ -27,7 +27,6 @@ bool aligned16(const void *p);
 bool partial_overlap(const void *a, const void *b, size_t bytes);
 int resolve_method(int method, int form, bool inverse, int *out_method, int *out_form);
 int device_form(int form);
-int grid_cap(const DevState &st, int method);
 const HostMatrices &host_matrices();
 const DeviceTables &host_device_tables();
 

@@ -55,7 +55,7 @@ def test_ragged_plan_small(tk, gpu, oracle, method):
     info = [ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_size_t()]
     assert tk.lib().tk_ragged_plan_info(plan, *[ctypes.byref(v) for v in info]) == 0
     assert info[3].value == 20000 and info[1].value in (16, 32)
-    print(f"ragged plan {method}: shape {info[1].value} grid {info[2].value}", flush=True)
+    print(f"ragged plan {method}: shape {info[1].value} oversub {info[2].value}", flush=True)
     assert tk.lib().tk_ragged_plan_execute(plan, _stream()) == 0
     torch.cuda.synchronize()
     got = d.cpu().numpy()
@@ -117,11 +117,15 @@ def test_ragged_full_size_digest(tk, gpu, digests):
     tk.lib().tk_ragged_plan_destroy(iplan)
 
 
-def test_autotune(tk, gpu):
-    m, w = ctypes.c_int(), ctypes.c_int()
-    assert tk.lib().tk_autotune(1 << 18, 0, ctypes.byref(m), ctypes.byref(w), _stream()) == 0
-    assert m.value in (tk.METHOD_MFMA16, tk.METHOD_MFMA32) and w.value >= 2
-    print(f"autotune fwd 2^18: method {m.value} waves/CU {w.value}", flush=True)
+@pytest.mark.parametrize("direction", [0, 1])
+def test_autotune(tk, gpu, direction):
+    m, w, g = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    assert tk.lib().tk_autotune(1 << 18, direction, ctypes.byref(m), ctypes.byref(w),
+                                ctypes.byref(g), _stream()) == 0
+    assert m.value in (tk.METHOD_MFMA16, tk.METHOD_MFMA32)
+    assert w.value in (2, 4) and g.value in (1, 2, 4, 8)
+    print(f"autotune dir {direction} 2^18: method {m.value} waves/WG {w.value} "
+          f"oversubscription {g.value}", flush=True)
 
 
 @pytest.mark.parametrize("k,ninst", [(2, 333), (3, 1000), (4, 257), (3, 1), (3, 17)])
