@@ -40,6 +40,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-check", action="store_true", help="skip the parity check")
     ap.add_argument("--only", default=None, help="time only this method (profiling runs)")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the ragged-plan and fused-matvec sections")
     ap.add_argument("--kernels-only", action="store_true",
                     help="profiling run: launch each kernel `steps` times plus a same-size "
                          "device copy, print per-kernel times, no CPU leg or parity check")
@@ -76,9 +78,15 @@ def time_method(tk, torch, method, src, buf, steps, warmup, npoly):
     def ev():
         return torch.cuda.Event(enable_timing=True)
 
-    for _ in range(warmup):
+    # Warm-up: `warmup` steps, then more until 0.5 s of work has run, so the
+    # GPU clocks have left their idle state before anything is timed.
+    t_w, n_w = time.perf_counter(), 0
+    while n_w < warmup or time.perf_counter() - t_w < 0.5:
         tk.ntt(src, out=buf, method=method)
         tk.invntt(buf, method=method)
+        n_w += 1
+        if n_w % 8 == 0:
+            torch.cuda.synchronize()
     torch.cuda.synchronize()
     e0, e1 = ev(), ev()
     t0 = time.perf_counter()
@@ -122,25 +130,119 @@ def time_method(tk, torch, method, src, buf, steps, warmup, npoly):
 
 
 def cpu_baseline(x):
-    """Reference C ntt()+invntt() round trip on the host cores (bounded sample)."""
+    """Reference C ntt()+invntt() round trip on the host cores (bounded sample):
+    all host threads, and one thread on a smaller sample."""
     from oracle.oracle import Oracle, Reference, reference_available
     kind = "reference" if reference_available() else "port"
     impl = Reference() if kind == "reference" else Oracle()
     cores = impl.max_threads()
-    npoly = min(x.shape[0], 1 << 18)
-    sample = np.ascontiguousarray(x[:npoly])
-    reps, t, polys = 0, 0.0, 0
-    while t < 3.0 and reps < 20:
-        a = sample.copy()
-        t0 = time.perf_counter()
-        impl.roundtrip_inplace(a)
-        t += time.perf_counter() - t0
-        polys += npoly
-        reps += 1
-    return {"value": polys / t, "unit": "poly/s", "cores": cores, "kind": kind,
-            "sample": f"{reps} x {npoly} polys of the same workload, ntt() then invntt() "
-                      f"per poly, OpenMP over polys on {cores} host threads, "
-                      f"{t:.2f} s wall ({t * cores:.1f} core-s)"}
+
+    def rate(npoly, nthreads, budget_s):
+        sample = np.ascontiguousarray(x[:npoly])
+        reps, t = 0, 0.0
+        while t < budget_s and reps < 20:
+            a = sample.copy()
+            t0 = time.perf_counter()
+            impl.roundtrip_inplace(a, nthreads)
+            t += time.perf_counter() - t0
+            reps += 1
+        return reps * npoly / t, reps, npoly, t
+
+    v, reps, n, t = rate(min(x.shape[0], 1 << 18), cores, 3.0)
+    v1, reps1, n1, t1 = rate(min(x.shape[0], 1 << 14), 1, 2.0)
+    return {"value": v, "unit": "poly/s", "cores": cores, "kind": kind,
+            "one_core_poly_s": v1,
+            "sample": f"{reps} x {n} polys of the same workload, ntt() then invntt() per "
+                      f"poly, OpenMP over polys on {cores} host threads, {t:.2f} s wall "
+                      f"({t * cores:.1f} core-s); one thread: {reps1} x {n1} polys, "
+                      f"{t1:.2f} s"}
+
+
+def bench_ragged(tk, torch, steps):
+    """Configuration 4: 2^20 polys in ragged ML-KEM requests (seed 4), forward
+    and inverse plans (autotuned persistent kernel, hipGraph replay)."""
+    import ctypes
+    N = 256
+    reqs, pool = tk.wl_ragged_layout(1 << 20, 4)
+    x = tk.wl_uniform(pool * N, 4).reshape(pool, N)
+    d = torch.from_numpy(x).to("cuda:0")
+    L = tk.lib()
+    plans = []
+    for direction in (0, 1):
+        table = (tk.TkRequest * len(reqs))()
+        for r, (off, n, k, kind) in enumerate(reqs):
+            table[r].base = d.data_ptr() + int(off) * N * 2
+            table[r].npoly, table[r].k, table[r].kind = int(n), int(k), int(kind)
+        p = ctypes.c_void_p()
+        rc = L.tk_ragged_plan_create(ctypes.byref(p), ctypes.addressof(table), len(reqs),
+                                     direction, tk.METHOD_AUTO, None)
+        if rc != 0:
+            raise SystemExit(f"tk_ragged_plan_create failed: {rc}")
+        plans.append(p)
+    info = [ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_size_t()]
+    L.tk_ragged_plan_info(plans[0], *[ctypes.byref(v) for v in info])
+    stream = torch.cuda.current_stream()
+    sp = ctypes.c_void_p(stream.cuda_stream)
+    for _ in range(2):
+        L.tk_ragged_plan_execute(plans[0], sp)
+        L.tk_ragged_plan_execute(plans[1], sp)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        L.tk_ragged_plan_execute(plans[0], sp)
+        L.tk_ragged_plan_execute(plans[1], sp)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    for p in plans:
+        L.tk_ragged_plan_destroy(p)
+    npoly = int(reqs[:, 1].sum())
+    return {"workload": f"{len(reqs)} requests of 1..4096 polys (rank-k vectors, 25% k x k "
+                        f"matrices), {npoly} polys in a {pool}-poly pool with gaps, seed 4",
+            "rt_poly_s": npoly / (ms / 1e3), "rt_ms": ms,
+            "method": {3: "mfma16", 4: "mfma32"}.get(info[0].value, info[0].value),
+            "oversub": info[2].value, "launch": "hipGraph replay of one persistent kernel"}
+
+
+def bench_matvec(tk, torch, steps, lat_launches=1000):
+    """Configuration 5: fused NTT -> basemul -> iNTT, 65,536 instances of rank
+    3 (seed 5), plus the launch latency at batch 1, 16, 256, 4096."""
+    N, k, ninst = 256, 3, 65536
+    A = tk.wl_uniform(ninst * k * k * N, 5, 0, 3329, stream=tk.WL_MAT)
+    s = tk.wl_cbd2(ninst * k * N, 5, stream=tk.WL_SEC)
+    dA = torch.from_numpy(A).to("cuda:0")
+    ds = torch.from_numpy(s).to("cuda:0")
+    dt = torch.empty_like(ds)
+    stream = torch.cuda.current_stream()
+    for _ in range(2):
+        tk.matvec(dA, ds, k, out=dt)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        tk.matvec(dA, ds, k, out=dt)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    bytes_per = (k * k + 2 * k) * N * 2
+    lat = {}
+    for b in (1, 16, 256, 4096):
+        a_b, s_b, t_b = dA[: b * k * k * N], ds[: b * k * N], dt[: b * k * N]
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+               for _ in range(lat_launches)]
+        for x0, x1 in evs:
+            x0.record(stream)
+            tk.matvec(a_b, s_b, k, out=t_b)
+            x1.record(stream)
+        torch.cuda.synchronize()
+        t = np.array([x0.elapsed_time(x1) * 1e3 for x0, x1 in evs])
+        lat[str(b)] = {"p50_us": float(np.percentile(t, 50)), "p99_us": float(np.percentile(t, 99))}
+    inst_s = ninst / (ms / 1e3)
+    return {"workload": "65,536 instances, k = 3, A_hat uniform [0,q), s CBD eta=2, seed 5",
+            "inst_s": inst_s, "ms": ms, "hbm_bytes_per_inst": bytes_per,
+            "roofline_frac": inst_s * bytes_per / 1e9 / HBM_PEAK_GBS,
+            "latency_us": lat, "latency_launches": lat_launches}
 
 
 def check_parity(tk, torch, method, x, buf_fwd, buf_rt):
@@ -219,6 +321,13 @@ def main():
         torch.cuda.synchronize()
         bit_exact = check_parity(tk, torch, main_m, x, fwd, rt)
 
+    extras = {}
+    if not args.no_extras:
+        print("[bench] config 4 ragged plan", file=sys.stderr, flush=True)
+        extras["config4_ragged"] = bench_ragged(tk, torch, max(args.steps // 3, 5))
+        print("[bench] config 5 fused matvec", file=sys.stderr, flush=True)
+        extras["config5_matvec"] = bench_matvec(tk, torch, max(args.steps // 3, 5))
+
     cpu = None
     if not args.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
@@ -257,6 +366,7 @@ def main():
         "bit_exact": bit_exact,
         "kernels": res,
         "cpu_baseline": cpu,
+        **extras,
     }
     line["copy_reference"] = copy_bandwidth(torch, src, buf, max(args.steps, 5))
     if "butterfly" in res and main_m != "butterfly":
