@@ -257,7 +257,11 @@ def check_parity(tk, torch, method, x, buf_fwd, buf_rt):
     return bool(ok_f and np.array_equal(inv, want))
 
 
-def load_traffic(kernel):
+def load_traffic(kernel, npoly):
+    """HBM bytes per launch of `kernel`, measured with rocprofv3 FETCH_SIZE and
+    WRITE_SIZE at 2^20 polys (profiles/traffic.json); None at other sizes."""
+    if npoly != 1 << 20:
+        return None
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
@@ -360,7 +364,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": load_traffic(kname),
+            "traffic": load_traffic(kname, npoly),
         },
         "rt_fraction_of_roofline": value * 2 * BYTES_PER_POLY_PASS / 1e9 / HBM_PEAK_GBS,
         "bit_exact": bit_exact,
