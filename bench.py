@@ -63,6 +63,18 @@ def copy_bandwidth(torch, src, dst, steps):
     return {"ms": ms, "GBs": 2 * src.numel() * 2 / (ms / 1e3) / 1e9}
 
 
+def warm(torch, fn, min_runs=2, min_s=0.5):
+    """Runs `fn` at least `min_runs` times and until `min_s` seconds of GPU
+    work have passed, so the clocks have left their idle state."""
+    t0, n = time.perf_counter(), 0
+    while n < min_runs or time.perf_counter() - t0 < min_s:
+        fn()
+        n += 1
+        if n % 8 == 0:
+            torch.cuda.synchronize()
+    torch.cuda.synchronize()
+
+
 def time_method(tk, torch, method, src, buf, steps, warmup, npoly):
     """Times the round trip and each kernel on the current stream.
 
@@ -80,14 +92,10 @@ def time_method(tk, torch, method, src, buf, steps, warmup, npoly):
 
     # Warm-up: `warmup` steps, then more until 0.5 s of work has run, so the
     # GPU clocks have left their idle state before anything is timed.
-    t_w, n_w = time.perf_counter(), 0
-    while n_w < warmup or time.perf_counter() - t_w < 0.5:
+    def step():
         tk.ntt(src, out=buf, method=method)
         tk.invntt(buf, method=method)
-        n_w += 1
-        if n_w % 8 == 0:
-            torch.cuda.synchronize()
-    torch.cuda.synchronize()
+    warm(torch, step, warmup)
     e0, e1 = ev(), ev()
     t0 = time.perf_counter()
     e0.record(stream)
@@ -183,24 +191,27 @@ def bench_ragged(tk, torch, steps):
     L.tk_ragged_plan_info(plans[0], *[ctypes.byref(v) for v in info])
     stream = torch.cuda.current_stream()
     sp = ctypes.c_void_p(stream.cuda_stream)
-    for _ in range(2):
-        L.tk_ragged_plan_execute(plans[0], sp)
-        L.tk_ragged_plan_execute(plans[1], sp)
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    for _ in range(steps):
-        L.tk_ragged_plan_execute(plans[0], sp)
-        L.tk_ragged_plan_execute(plans[1], sp)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / steps
+
+    def timed(run):
+        warm(torch, lambda: (run(plans[0], sp), run(plans[1], sp)))
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(steps):
+            run(plans[0], sp)
+            run(plans[1], sp)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / steps
+
+    ms = timed(L.tk_ragged_plan_execute)
+    ms_direct = timed(L.tk_ragged_plan_launch)
     for p in plans:
         L.tk_ragged_plan_destroy(p)
     npoly = int(reqs[:, 1].sum())
     return {"workload": f"{len(reqs)} requests of 1..4096 polys (rank-k vectors, 25% k x k "
                         f"matrices), {npoly} polys in a {pool}-poly pool with gaps, seed 4",
             "rt_poly_s": npoly / (ms / 1e3), "rt_ms": ms,
+            "rt_ms_direct_launch": ms_direct,
             "method": {3: "mfma16", 4: "mfma32"}.get(info[0].value, info[0].value),
             "oversub": info[2].value, "launch": "hipGraph replay of one persistent kernel"}
 
@@ -215,9 +226,7 @@ def bench_matvec(tk, torch, steps, lat_launches=1000):
     ds = torch.from_numpy(s).to("cuda:0")
     dt = torch.empty_like(ds)
     stream = torch.cuda.current_stream()
-    for _ in range(2):
-        tk.matvec(dA, ds, k, out=dt)
-    torch.cuda.synchronize()
+    warm(torch, lambda: tk.matvec(dA, ds, k, out=dt))
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record(stream)
     for _ in range(steps):
@@ -328,9 +337,9 @@ def main():
     extras = {}
     if not args.no_extras:
         print("[bench] config 4 ragged plan", file=sys.stderr, flush=True)
-        extras["config4_ragged"] = bench_ragged(tk, torch, max(args.steps // 3, 5))
+        extras["config4_ragged"] = bench_ragged(tk, torch, args.steps)
         print("[bench] config 5 fused matvec", file=sys.stderr, flush=True)
-        extras["config5_matvec"] = bench_matvec(tk, torch, max(args.steps // 3, 5))
+        extras["config5_matvec"] = bench_matvec(tk, torch, args.steps)
 
     cpu = None
     if not args.no_cpu:

@@ -34,14 +34,22 @@ __device__ __forceinline__ int32_t bias_adjust()
   return FORM == kFormCentred ? 0 : -kCentre;
 }
 
-// Per-tile poly addresses.  Lanes 0..P-1 compute one poly address each
-// (linear: from the tile index; ragged: by walking the request table) and
-// park them in LDS, where the staging loops read them back; keeping them out
-// of vector registers across the two passes saves about 40 VGPRs, which is
-// one more resident wave per SIMD.
+// Per-tile poly addresses.  Lanes 0..P-1 hold one poly address each and park
+// them in LDS, where the staging loops read them back; keeping them out of
+// vector registers across the two passes saves about 40 VGPRs, which is one
+// more resident wave per SIMD.  Linear tiles compute the addresses from the
+// tile index.  Ragged tiles read them from the plan's address table one tile
+// ahead (tile_fetch), so the read is in flight while the current tile is
+// staged and never stalls the wave on its own.
+template <int P>
+__device__ __forceinline__ int16_t *tile_fetch(const LinearTiles &, size_t, int)
+{
+  return nullptr;
+}
+
 template <int P>
 __device__ __forceinline__ void tile_prepare(const LinearTiles &io, size_t tile, int lane,
-                                             int16_t **lds)
+                                             int16_t *, int16_t **lds)
 {
   if (lane < P) {
     const size_t p = tile * P + lane;
@@ -66,20 +74,18 @@ __device__ __forceinline__ int16_t *tile_dst(const LinearTiles &, size_t, int pp
 }
 
 template <int P>
-__device__ __forceinline__ void tile_prepare(const RaggedTiles &io, size_t tile, int lane,
+__device__ __forceinline__ int16_t *tile_fetch(const RaggedTiles &io, size_t tile, int lane)
+{
+  const uint64_t g = (uint64_t)tile * P + lane;
+  return lane < P && g < io.npoly ? io.ptrs[g] : nullptr;
+}
+
+template <int P>
+__device__ __forceinline__ void tile_prepare(const RaggedTiles &, size_t, int lane, int16_t *pre,
                                              int16_t **lds)
 {
-  if (lane < P) {
-    const uint64_t g = (uint64_t)tile * P + lane;
-    int16_t *p = nullptr;
-    if (g < io.npoly) {
-      uint32_t r = io.tile_req[tile];
-      while (io.first[r + 1] <= g)
-        r++;
-      p = io.base[r] + (size_t)(g - io.first[r]) * kN;
-    }
-    lds[lane] = p;
-  }
+  if (lane < P)
+    lds[lane] = pre;
   wave_sync();
 }
 
@@ -135,9 +141,12 @@ __global__ __launch_bounds__(64 * WAVES) void k_mfma16(Tiles io,
   // a register prefetch of the next tile measured slower (it costs a wave of
   // occupancy).
   const size_t ntiles = (io.npoly + P - 1) / P;
-  for (size_t tile = (size_t)blockIdx.x * WAVES + w; tile < ntiles;
-       tile += (size_t)gridDim.x * WAVES) {
-    tile_prepare<P>(io, tile, lane, s_ptr[w]);
+  const size_t stride = (size_t)gridDim.x * WAVES;
+  size_t tile = (size_t)blockIdx.x * WAVES + w;
+  int16_t *pre = tile_fetch<P>(io, tile, lane);
+  for (; tile < ntiles; tile += stride) {
+    tile_prepare<P>(io, tile, lane, pre, s_ptr[w]);
+    pre = tile_fetch<P>(io, tile + stride, lane);
     tile_in<P>(L, [&](int pp) { return tile_src<P>(io, tile, pp, s_ptr[w]); }, lane);
     wave_sync();
     if (!INV) {
@@ -190,9 +199,12 @@ __global__ __launch_bounds__(64 * WAVES) void k_mfma32(Tiles io,
   int16_t *Pp = L + t * kStride;
   // 32 polys per wave: no register budget left for a prefetch buffer.
   const size_t ntiles = (io.npoly + P - 1) / P;
-  for (size_t tile = (size_t)blockIdx.x * WAVES + w; tile < ntiles;
-       tile += (size_t)gridDim.x * WAVES) {
-    tile_prepare<P>(io, tile, lane, s_ptr[w]);
+  const size_t stride = (size_t)gridDim.x * WAVES;
+  size_t tile = (size_t)blockIdx.x * WAVES + w;
+  int16_t *pre = tile_fetch<P>(io, tile, lane);
+  for (; tile < ntiles; tile += stride) {
+    tile_prepare<P>(io, tile, lane, pre, s_ptr[w]);
+    pre = tile_fetch<P>(io, tile + stride, lane);
     tile_in<P>(L, [&](int pp) { return tile_src<P>(io, tile, pp, s_ptr[w]); }, lane);
     wave_sync();
     if (!INV) {

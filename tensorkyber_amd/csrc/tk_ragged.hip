@@ -2,14 +2,15 @@
 // one launch) and the launch-shape autotuner.
 //
 // A plan flattens the request table into a global poly numbering, uploads
-// the table (bases, prefix sums) and a tile -> first-request map once, picks
-// the MFMA tile shape / waves per workgroup / grid with the autotuner, and
-// records the persistent kernel into a hipGraph.  Executing the plan replays
+// the device address of every poly once, picks the MFMA tile shape / waves
+// per workgroup / grid with the autotuner, and records the persistent kernel
+// into a hipGraph.  Executing the plan replays
 // the graph: one launch whatever the number of requests, no host work.
 #include <hip/hip_runtime.h>
 #include <string.h>
 
 #include <map>
+#include <new>
 #include <mutex>
 #include <tuple>
 #include <vector>
@@ -222,30 +223,18 @@ int tk_ragged_plan_create(tk_ragged_plan **plan, const tk_request *table, size_t
   }
   const int shape = tv.method == TK_METHOD_MFMA32 ? 32 : 16;
 
-  // Host images of the device tables.
-  const size_t ntiles = (total + shape - 1) / shape;
-  std::vector<int16_t *> base(nreq);
-  std::vector<uint64_t> first(nreq + 1);
-  std::vector<uint32_t> tile_req(ntiles);
-  uint64_t acc = 0;
-  for (size_t r = 0; r < nreq; r++) {
-    base[r] = table[r].base;
-    first[r] = acc;
-    acc += table[r].npoly;
+  // Host image of the address table: poly g of the flattened numbering.
+  std::vector<int16_t *> ptrs;
+  try {
+    ptrs.resize(total);
+  } catch (const std::bad_alloc &) {
+    return TK_ENOMEM;
   }
-  first[nreq] = acc;
-  size_t r = 0;
-  for (size_t t = 0; t < ntiles; t++) {
-    const uint64_t g = (uint64_t)t * shape;
-    while (first[r + 1] <= g)
-      r++;
-    tile_req[t] = (uint32_t)r;
-  }
-  const size_t b_base = nreq * sizeof(int16_t *);
-  const size_t b_first = (nreq + 1) * sizeof(uint64_t);
-  const size_t b_tile = ntiles * sizeof(uint32_t);
-  const size_t off_first = (b_base + 255) & ~(size_t)255;
-  const size_t off_tile = (off_first + b_first + 255) & ~(size_t)255;
+  size_t g = 0;
+  for (size_t r = 0; r < nreq; r++)
+    for (uint32_t i = 0; i < table[r].npoly; i++)
+      ptrs[g++] = table[r].base + (size_t)i * kN;
+  const size_t b_ptrs = total * sizeof(int16_t *);
 
   tk_ragged_plan *p = new tk_ragged_plan();
   p->device = st->device;
@@ -257,19 +246,15 @@ int tk_ragged_plan_create(tk_ragged_plan **plan, const tk_request *table, size_t
   p->npoly = total;
   p->nreq = nreq;
   p->tab = st->tables;
-  if (hipMalloc(&p->dmem, off_tile + b_tile) != hipSuccess) {
+  if (hipMalloc(&p->dmem, b_ptrs) != hipSuccess) {
     plan_free(p);
     return TK_ENOMEM;
   }
-  char *d = (char *)p->dmem;
-  if ((rc = record_hip(hipMemcpy(d, base.data(), b_base, hipMemcpyHostToDevice))) ||
-      (rc = record_hip(hipMemcpy(d + off_first, first.data(), b_first, hipMemcpyHostToDevice))) ||
-      (rc = record_hip(hipMemcpy(d + off_tile, tile_req.data(), b_tile, hipMemcpyHostToDevice)))) {
+  if ((rc = record_hip(hipMemcpy(p->dmem, ptrs.data(), b_ptrs, hipMemcpyHostToDevice)))) {
     plan_free(p);
     return rc;
   }
-  p->tiles = RaggedTiles{(int16_t *const *)d, (const uint64_t *)(d + off_first),
-                         (const uint32_t *)(d + off_tile), total};
+  p->tiles = RaggedTiles{(int16_t *const *)p->dmem, total};
 
   // Record the launch into a graph on a private capture stream.
   hipStream_t cap = nullptr;
