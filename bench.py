@@ -220,10 +220,9 @@ def bench_matvec(tk, torch, steps, lat_launches=1000):
     """Configuration 5: fused NTT -> basemul -> iNTT, 65,536 instances of rank
     3 (seed 5), plus the launch latency at batch 1, 16, 256, 4096."""
     N, k, ninst = 256, 3, 65536
-    A = tk.wl_uniform(ninst * k * k * N, 5, 0, 3329, stream=tk.WL_MAT)
-    s = tk.wl_cbd2(ninst * k * N, 5, stream=tk.WL_SEC)
-    dA = torch.from_numpy(A).to("cuda:0")
-    ds = torch.from_numpy(s).to("cuda:0")
+    A, s = tk.matvec_workload(ninst, k)
+    dA = torch.from_numpy(A.reshape(-1)).to("cuda:0")
+    ds = torch.from_numpy(s.reshape(-1)).to("cuda:0")
     dt = torch.empty_like(ds)
     stream = torch.cuda.current_stream()
     warm(torch, lambda: tk.matvec(dA, ds, k, out=dt))
@@ -252,6 +251,54 @@ def bench_matvec(tk, torch, steps, lat_launches=1000):
             "inst_s": inst_s, "ms": ms, "hbm_bytes_per_inst": bytes_per,
             "roofline_frac": inst_s * bytes_per / 1e9 / HBM_PEAK_GBS,
             "latency_us": lat, "latency_launches": lat_launches}
+
+
+def bench_packed(tk, torch, steps, npoly=1 << 20):
+    """Packed small-coefficient input (the TC-PACK analogue): forward NTT of
+    2^20 packed polys at 2 bits (ternary) and 3 bits (CBD eta = 2), against
+    the int16 forward on the same values; and configuration 5 with its CBD
+    secrets packed in 3 bits."""
+    stream = torch.cuda.current_stream()
+    out = {}
+
+    def per_launch(fn):
+        warm(torch, fn)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(steps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / steps
+
+    for bits in (2, 3):
+        x = torch.from_numpy(tk.packed_workload(bits, npoly)).to("cuda:0")
+        packed = tk.pack_small(x, bits)
+        dst = torch.empty_like(x)
+        ms = per_launch(lambda: tk.ntt_packed(packed, bits, out=dst))
+        ms16 = per_launch(lambda: tk.ntt(x, out=dst))
+        ms_pack = per_launch(lambda: tk.pack_small(x, bits, out=packed))
+        nbytes = tk.packed_poly_bytes(bits) + 512
+        out[f"bits{bits}"] = {
+            "npoly": npoly, "fwd_us": ms * 1e3, "poly_s": npoly / (ms / 1e3),
+            "bytes_per_poly": nbytes,
+            "roofline_frac": npoly * nbytes / (ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+            "int16_fwd_us_same_values": ms16 * 1e3,
+            "speedup_vs_int16_fwd": ms16 / ms,
+            "pack_kernel_us": ms_pack * 1e3,
+        }
+    A, s = tk.matvec_workload()
+    k, ninst = 3, s.shape[0]
+    dA = torch.from_numpy(A).to("cuda:0")
+    sp = tk.pack_small(torch.from_numpy(s.reshape(-1, 256)).to("cuda:0"), 3)
+    t = torch.empty((ninst * k, 256), dtype=torch.int16, device="cuda:0")
+    ms = per_launch(lambda: tk.matvec_packed(dA, sp, k, 3, out=t))
+    nbytes = (k * k + k) * 512 + k * tk.packed_poly_bytes(3)
+    out["config5_s_packed3"] = {
+        "ninst": ninst, "ms": ms, "inst_s": ninst / (ms / 1e3), "bytes_per_inst": nbytes,
+        "roofline_frac": ninst * nbytes / (ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+    }
+    return out
 
 
 def check_parity(tk, torch, method, x, buf_fwd, buf_rt):
@@ -340,6 +387,8 @@ def main():
         extras["config4_ragged"] = bench_ragged(tk, torch, args.steps)
         print("[bench] config 5 fused matvec", file=sys.stderr, flush=True)
         extras["config5_matvec"] = bench_matvec(tk, torch, args.steps)
+        print("[bench] packed input", file=sys.stderr, flush=True)
+        extras["packed_input"] = bench_packed(tk, torch, args.steps)
 
     cpu = None
     if not args.no_cpu:

@@ -110,8 +110,8 @@ typedef struct {
   uint16_t kind;
 } tk_request;
 
-/* Host-built table of nreq requests; the plan copies it, and its tile map, to
- * device memory once, picks the kernel shape with the autotuner (or takes
+/* Host-built table of nreq requests; the plan copies the device address of
+ * every requested poly to device memory once, picks the kernel shape with the autotuner (or takes
  * method MFMA16 / MFMA32 as given) and records the launch in a hipGraph.
  * direction: 0 = forward (canonical out), 1 = inverse (centred out).
  * Transforms are in place.  Plan creation allocates and synchronises; the
@@ -138,6 +138,35 @@ int tk_ragged_plan_destroy(tk_ragged_plan *plan);
  * NTT of s and the products stay in registers/LDS.  Output is centred. */
 int matvec_ntt_fused(int16_t *t, const int16_t *A_hat, const int16_t *s, int k, size_t ninst,
                      void *stream);
+
+/* ---- packed small-coefficient input ------------------------------------ */
+
+/* The packed form of the reference's TC-PACK path (ntt_bench.cu:278-391,
+ * 740-773): a poly of small coefficients takes bits-bit two's-complement
+ * fields, 8/bits per byte, coefficient i in byte i/(8/bits) at bit
+ * bits*(i%(8/bits)), little-endian within the byte (pack_small,
+ * ntt_bench.cu:749-757).  bits 2 (ternary), 3 (CBD eta = 2 or 3) or 4; a poly
+ * takes tk_packed_poly_bytes(bits) = 64, 128 or 128 bytes, polys back to back.
+ * Values outside [-2^(bits-1), 2^(bits-1)) keep their low bits, as in
+ * pack_small. */
+size_t tk_packed_poly_bytes(int bits); /* 0 for an unsupported width */
+
+/* Packs npoly int16 polys (device) into the packed form (device, 16-byte
+ * aligned, not overlapping in). */
+int pack_small_batch(uint8_t *packed, const int16_t *in, size_t npoly, int bits, void *stream);
+
+/* Forward NTT of packed polys (k_ntt_tc_pack, ntt_bench.cu:352-391): unpacked
+ * while staged, then transformed as ntt_batch_ex does.  method AUTO, MFMA,
+ * MFMA16 or MFMA32 (BUTTERFLY: TK_ENOTSUP); form DEFAULT/CANONICAL or
+ * CENTERED.  out must not overlap packed. */
+int ntt_batch_packed(int16_t *out, const uint8_t *packed, size_t npoly, int bits, int method,
+                     int form, void *stream);
+
+/* matvec_ntt_fused with s in the packed form: s_packed is
+ * [ninst][k][tk_packed_poly_bytes(bits)].  t must not overlap A_hat or
+ * s_packed.  Output is centred. */
+int matvec_ntt_fused_packed(int16_t *t, const int16_t *A_hat, const uint8_t *s_packed, int k,
+                            size_t ninst, int bits, void *stream);
 
 /* ---- autotuner ---------------------------------------------------------- */
 

@@ -263,3 +263,47 @@ long long oracle_check_reductions(void)
   }
   return bad;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Packed small-coefficient input (ntt_bench/ntt_bench.cu:749-757 packs on the
+ * host; the device unpack at :325-348 sign-extends each field). */
+
+size_t oracle_packed_poly_bytes(int bits)
+{
+  if (bits < 2 || bits > 4)
+    return 0;
+  return (size_t)ORACLE_N / (size_t)(8 / bits);
+}
+
+void oracle_pack_small(uint8_t *packed, const int16_t *polys, size_t npoly, int bits)
+{
+  const size_t ppb = oracle_packed_poly_bytes(bits);
+  const int per = 8 / bits;
+  if (ppb == 0)
+    return;
+  for (size_t p = 0; p < npoly; p++) {
+    uint8_t *dst = packed + p * ppb;
+    const int16_t *src = polys + p * ORACLE_N;
+    for (size_t b = 0; b < ppb; b++)
+      dst[b] = 0;
+    for (int i = 0; i < ORACLE_N; i++)
+      dst[i / per] |= (uint8_t)((src[i] & ((1 << bits) - 1)) << ((i % per) * bits));
+  }
+}
+
+void oracle_unpack_small(int16_t *polys, const uint8_t *packed, size_t npoly, int bits)
+{
+  const size_t ppb = oracle_packed_poly_bytes(bits);
+  const int per = 8 / bits;
+  if (ppb == 0)
+    return;
+  for (size_t p = 0; p < npoly; p++) {
+    const uint8_t *src = packed + p * ppb;
+    int16_t *dst = polys + p * ORACLE_N;
+    for (int i = 0; i < ORACLE_N; i++) {
+      const int code = (src[i / per] >> ((i % per) * bits)) & ((1 << bits) - 1);
+      /* sign-extend the bits-wide field */
+      dst[i] = (int16_t)(code >= (1 << (bits - 1)) ? code - (1 << bits) : code);
+    }
+  }
+}

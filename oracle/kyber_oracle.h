@@ -20,6 +20,8 @@
  *   ntt_bench/ref/ntt.c:80-95     ntt                oracle_ntt
  *   ntt_bench/ref/ntt.c:106-126   invntt             oracle_invntt
  *   ntt_bench/ref/ntt.c:139-146   basemul            oracle_basemul
+ *   ntt_bench/ntt_bench.cu:749-757 pack_small        oracle_pack_small
+ *   ntt_bench/ntt_bench.cu:325-348 pack_unpack       oracle_unpack_small
  *
  * Parity pins (tests/test_oracle.py): raw equality with the compiled
  * reference (oracle/_ref) and with the committed golden fixtures
@@ -73,6 +75,15 @@ void oracle_roundtrip_batch(int16_t *polys, size_t npoly, int nthreads);
  * reduce of every coefficient, then invntt. */
 void oracle_matvec_batch(int16_t *t, const int16_t *A_hat, const int16_t *s, int k,
                          size_t ninst, int nthreads);
+
+/* Packed small-coefficient polys (the TC-PACK input format): bits-bit
+ * two's-complement fields, per = 8/bits per byte, coefficient i in byte
+ * i/per at bit bits*(i%per).  A poly takes 256/per bytes (bits 2: 64;
+ * bits 3, 4: 128).  pack keeps the low bits of each value; unpack
+ * sign-extends each field.  Batched over npoly polys back to back. */
+size_t oracle_packed_poly_bytes(int bits);
+void oracle_pack_small(uint8_t *packed, const int16_t *polys, size_t npoly, int bits);
+void oracle_unpack_small(int16_t *polys, const uint8_t *packed, size_t npoly, int bits);
 
 /* Canonical representative in [0, q). */
 static inline int16_t oracle_canon(int16_t v)

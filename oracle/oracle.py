@@ -109,8 +109,31 @@ class Oracle(_Batch):
                                           ctypes.c_int]
         L.oracle_matvec_batch.restype = None
         L.oracle_fqmul.argtypes = [ctypes.c_int16, ctypes.c_int16]
+        L.oracle_packed_poly_bytes.argtypes = [ctypes.c_int]
+        L.oracle_packed_poly_bytes.restype = ctypes.c_size_t
+        L.oracle_pack_small.argtypes = [ctypes.c_void_p, _i16p, ctypes.c_size_t, ctypes.c_int]
+        L.oracle_unpack_small.argtypes = [_i16p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
         L.oracle_fqmul.restype = ctypes.c_int16
         L.oracle_check_reductions.restype = ctypes.c_longlong
+
+    def packed_poly_bytes(self, bits: int) -> int:
+        return int(self.lib.oracle_packed_poly_bytes(bits))
+
+    def pack_small(self, polys, bits: int) -> np.ndarray:
+        a = np.ascontiguousarray(polys, dtype=np.int16).reshape(-1, 256)
+        ppb = self.packed_poly_bytes(bits)
+        assert ppb > 0, f"unsupported width {bits}"
+        out = np.zeros((a.shape[0], ppb), dtype=np.uint8)
+        self.lib.oracle_pack_small(out.ctypes.data, _ptr(a), a.shape[0], bits)
+        return out
+
+    def unpack_small(self, packed, bits: int) -> np.ndarray:
+        ppb = self.packed_poly_bytes(bits)
+        assert ppb > 0, f"unsupported width {bits}"
+        b = np.ascontiguousarray(packed, dtype=np.uint8).reshape(-1, ppb)
+        out = np.zeros((b.shape[0], 256), dtype=np.int16)
+        self.lib.oracle_unpack_small(_ptr(out), b.ctypes.data, b.shape[0], bits)
+        return out
 
     def check_reductions(self) -> int:
         """Violations of the reductions' contracts over their whole domains."""

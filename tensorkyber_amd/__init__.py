@@ -89,8 +89,10 @@ def _declare(lib):
         "tk_wl_cbd2": ([vp, sz, u64, u64], None),
         "tk_wl_ragged_layout": ([vp, sz, sz, u64, ctypes.POINTER(sz)], sz),
         "tk_wl_canon_digest": ([vp, sz], u64),
-    }
-    optional = {
+        "tk_packed_poly_bytes": ([i], sz),
+        "pack_small_batch": ([vp, vp, sz, i, vp], i),
+        "ntt_batch_packed": ([vp, vp, sz, i, i, i, vp], i),
+        "matvec_ntt_fused_packed": ([vp, vp, vp, i, sz, i, vp], i),
         "tk_ragged_plan_create": ([ctypes.POINTER(vp), vp, sz, i, i, vp], i),
         "tk_ragged_plan_execute": ([vp, vp], i),
         "tk_ragged_plan_launch": ([vp, vp], i),
@@ -103,10 +105,6 @@ def _declare(lib):
     for name, (args, res) in sigs.items():
         f = getattr(lib, name)
         f.argtypes, f.restype = args, res
-    for name, (args, res) in optional.items():
-        if hasattr(lib, name):
-            f = getattr(lib, name)
-            f.argtypes, f.restype = args, res
     return lib
 
 
@@ -212,6 +210,70 @@ def matvec(A_hat, s, k: int, out=None, stream=None):
     return out
 
 
+def packed_poly_bytes(bits: int) -> int:
+    """Bytes of one packed poly of `bits`-bit coefficients (0 if unsupported)."""
+    return int(lib().tk_packed_poly_bytes(bits))
+
+
+def _as_packed(t, bits: int, name="packed"):
+    torch = _torch()
+    ppb = packed_poly_bytes(bits)
+    if ppb == 0:
+        raise ValueError(f"unsupported coefficient width {bits}")
+    if not isinstance(t, torch.Tensor) or not t.is_cuda or t.dtype != torch.uint8:
+        raise TypeError(f"{name} must be a uint8 CUDA tensor")
+    if not t.is_contiguous() or t.numel() % ppb:
+        raise ValueError(f"{name} must be contiguous with a multiple of {ppb} bytes")
+    return t, ppb
+
+
+def pack_small(polys, bits: int, out=None, stream=None):
+    """Packs small-coefficient int16 polys into `bits`-bit fields (uint8)."""
+    polys = _as_polys(polys)
+    npoly = polys.numel() // N
+    if out is None:
+        ppb = packed_poly_bytes(bits)
+        if ppb == 0:
+            raise ValueError(f"unsupported coefficient width {bits}")
+        out = _torch().empty(npoly * ppb, dtype=_torch().uint8, device=polys.device)
+    out, ppb = _as_packed(out, bits, "out")
+    if out.numel() != npoly * ppb:
+        raise ValueError("out has the wrong size")
+    _check(lib().pack_small_batch(out.data_ptr(), polys.data_ptr(), npoly, bits,
+                                  _stream_ptr(stream)), "pack_small_batch")
+    return out
+
+
+def ntt_packed(packed, bits: int, out=None, method="auto", form="default", stream=None):
+    """Forward NTT of packed small-coefficient polys into int16 [npoly, 256]."""
+    packed, ppb = _as_packed(packed, bits)
+    npoly = packed.numel() // ppb
+    torch = _torch()
+    out = torch.empty((npoly, N), dtype=torch.int16, device=packed.device) if out is None \
+        else _as_polys(out, "out")
+    if out.numel() != npoly * N:
+        raise ValueError("out has the wrong size")
+    _check(lib().ntt_batch_packed(out.data_ptr(), packed.data_ptr(), npoly, bits, _method(method),
+                                  _form(form), _stream_ptr(stream)), "ntt_batch_packed")
+    return out
+
+
+def matvec_packed(A_hat, s_packed, k: int, bits: int, out=None, stream=None):
+    """Fused t = invntt(A_hat o ntt(s)) with s packed: [ninst, k, packed bytes]."""
+    A_hat = _as_polys(A_hat, "A_hat")
+    s_packed, ppb = _as_packed(s_packed, bits, "s_packed")
+    ninst = s_packed.numel() // (k * ppb)
+    if s_packed.numel() != ninst * k * ppb or A_hat.numel() != ninst * k * k * N:
+        raise ValueError("A_hat and s_packed shapes disagree")
+    torch = _torch()
+    out = torch.empty((ninst * k, N), dtype=torch.int16, device=A_hat.device) if out is None \
+        else _as_polys(out, "out")
+    _check(lib().matvec_ntt_fused_packed(out.data_ptr(), A_hat.data_ptr(), s_packed.data_ptr(), k,
+                                         ninst, bits, _stream_ptr(stream)),
+           "matvec_ntt_fused_packed")
+    return out
+
+
 # ---------------------------------------------------------------------------
 # host introspection and synthetic workloads (no device needed)
 # ---------------------------------------------------------------------------
@@ -293,8 +355,33 @@ def workload(config: int, npoly: int | None = None):
     raise ValueError(config)
 
 
+def matvec_workload(ninst: int = 65536, k: int = 3):
+    """Configuration 5: A_hat uniform [0, q) (NTT domain) and CBD eta = 2
+    secrets, seed 5; returns ([ninst, k, k, 256], [ninst, k, 256])."""
+    A = wl_uniform(ninst * k * k * N, 5, 0, Q, stream=WL_MAT).reshape(ninst, k, k, N)
+    s = wl_cbd2(ninst * k * N, 5, stream=WL_SEC).reshape(ninst, k, N)
+    return A, s
+
+
+def packed_workload(bits: int, npoly: int = 1 << 20) -> np.ndarray:
+    """Small-coefficient polys for the packed forward path: ternary (seed 6)
+    for 2 bits, CBD eta = 2 (seed 7) for 3 bits, uniform [-8, 8) (seed 8) for
+    4 bits."""
+    if bits == 2:
+        x = wl_uniform(npoly * N, 6, -1, 2)
+    elif bits == 3:
+        x = wl_cbd2(npoly * N, 7)
+    elif bits == 4:
+        x = wl_uniform(npoly * N, 8, -8, 8)
+    else:
+        raise ValueError(bits)
+    return x.reshape(npoly, N)
+
+
 __all__ = [
     "N", "Q", "TkError", "load_library", "lib", "ntt", "invntt", "basemul", "matvec",
-    "host_tables", "host_mfma_operands", "workload", "wl_uniform", "wl_edge_mix", "wl_cbd2",
-    "wl_ragged_layout", "canon_digest", "TkRequest", "METHODS", "FORMS",
+    "packed_poly_bytes", "pack_small", "ntt_packed", "matvec_packed",
+    "host_tables", "host_mfma_operands", "workload", "matvec_workload", "packed_workload",
+    "wl_uniform", "wl_edge_mix", "wl_cbd2", "wl_ragged_layout", "canon_digest", "TkRequest",
+    "METHODS", "FORMS",
 ]

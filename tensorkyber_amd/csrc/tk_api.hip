@@ -285,7 +285,84 @@ int matvec_ntt_fused(int16_t *t, const int16_t *A_hat, const int16_t *s, int k, 
   int rc = init_device(&st);
   if (rc)
     return rc;
-  if (launch_matvec(k, t, A_hat, s, ninst, st->tables, st->ncu, (hipStream_t)stream))
+  if (launch_matvec(k, 0, t, A_hat, s, ninst, st->tables, st->ncu, (hipStream_t)stream))
+    return TK_ENOTSUP;
+  return after_launch();
+}
+
+size_t tk_packed_poly_bytes(int bits)
+{
+  return (bits >= 2 && bits <= 4) ? (size_t)(kN / (8 / bits)) : 0;
+}
+
+int pack_small_batch(uint8_t *packed, const int16_t *in, size_t npoly, int bits, void *stream)
+{
+  const size_t ppb = tk_packed_poly_bytes(bits);
+  if (ppb == 0)
+    return TK_EINVAL;
+  if (npoly == 0)
+    return TK_OK;
+  if (!aligned16(packed) || !aligned16(in) || npoly > kMaxPolys)
+    return TK_EINVAL;
+  const uintptr_t pp = (uintptr_t)packed, ip = (uintptr_t)in;
+  if (pp < ip + npoly * kN * sizeof(int16_t) && ip < pp + npoly * ppb)
+    return TK_EINVAL;
+  DevState *st = nullptr;
+  int rc = init_device(&st);
+  if (rc)
+    return rc;
+  if (launch_pack_small(bits, packed, in, npoly, st->ncu, (hipStream_t)stream))
+    return TK_ENOTSUP;
+  return after_launch();
+}
+
+int ntt_batch_packed(int16_t *out, const uint8_t *packed, size_t npoly, int bits, int method,
+                     int form, void *stream)
+{
+  const size_t ppb = tk_packed_poly_bytes(bits);
+  if (ppb == 0)
+    return TK_EINVAL;
+  int m = 0, f = 0;
+  int rc = resolve_method(method == TK_METHOD_AUTO ? TK_METHOD_MFMA : method, form, false, &m, &f);
+  if (rc)
+    return rc;
+  if (m == TK_METHOD_BUTTERFLY)
+    return TK_ENOTSUP;
+  if (npoly == 0)
+    return TK_OK;
+  if (!aligned16(out) || !aligned16(packed) || npoly > kMaxPolys)
+    return TK_EINVAL;
+  const uintptr_t op = (uintptr_t)out, pp = (uintptr_t)packed;
+  if (op < pp + npoly * ppb && pp < op + npoly * kN * sizeof(int16_t))
+    return TK_EINVAL;
+  DevState *st = nullptr;
+  if ((rc = init_device(&st)))
+    return rc;
+  if (launch_mfma_packed(m == TK_METHOD_MFMA32 ? 32 : 16, bits, device_form(f), out, packed, npoly,
+                         st->tables, LaunchShape{st->ncu, 4, kDefaultOversub}, (hipStream_t)stream))
+    return TK_ENOTSUP;
+  return after_launch();
+}
+
+int matvec_ntt_fused_packed(int16_t *t, const int16_t *A_hat, const uint8_t *s_packed, int k,
+                            size_t ninst, int bits, void *stream)
+{
+  const size_t ppb = tk_packed_poly_bytes(bits);
+  if (ppb == 0 || k < 2 || k > 4)
+    return TK_EINVAL;
+  if (ninst == 0)
+    return TK_OK;
+  if (!aligned16(t) || !aligned16(A_hat) || !aligned16(s_packed) || ninst > kMaxPolys / 16)
+    return TK_EINVAL;
+  const size_t vb = ninst * k * kN * sizeof(int16_t), mb = vb * k, sb = ninst * k * ppb;
+  const uintptr_t tp = (uintptr_t)t, ap = (uintptr_t)A_hat, sp = (uintptr_t)s_packed;
+  if ((tp < ap + mb && ap < tp + vb) || (tp < sp + sb && sp < tp + vb))
+    return TK_EINVAL;
+  DevState *st = nullptr;
+  int rc = init_device(&st);
+  if (rc)
+    return rc;
+  if (launch_matvec(k, bits, t, A_hat, s_packed, ninst, st->tables, st->ncu, (hipStream_t)stream))
     return TK_ENOTSUP;
   return after_launch();
 }

@@ -27,6 +27,8 @@
 // any int16 and b, gamma*b1 lie in [0, 2q), so |sum| <= k * 32768 * (q-1 +
 // 2q-1) < 1.31e9 < 2^31 for k <= 4; it is reduced once per output
 // coefficient.
+#include <type_traits>
+
 #include "tk_launch.h"
 #include "tk_mfma_core.h"
 
@@ -61,10 +63,12 @@ __device__ __forceinline__ long ld_a(const uint32_t (*m)[64][2], int limb, int l
   return *reinterpret_cast<const long *>(m[limb][lane]);
 }
 
-template <int K>
+// SBITS = 0: s holds int16 polys [ninst][K][256]; SBITS = 2, 3, 4: s holds
+// the packed small-coefficient form [ninst][K][packed bytes].
+template <int K, int SBITS>
 __global__ __launch_bounds__(64 * K) void k_matvec(int16_t *__restrict__ t,
                                                    const int16_t *__restrict__ A,
-                                                   const int16_t *__restrict__ s, size_t ninst,
+                                                   const void *__restrict__ s, size_t ninst,
                                                    const DeviceTables *__restrict__ tab)
 {
   constexpr int P = kTileInst;
@@ -109,11 +113,23 @@ __global__ __launch_bounds__(64 * K) void k_matvec(int16_t *__restrict__ t,
     const int nvalid = (int)(ninst - i0 < (size_t)P ? ninst - i0 : P);
 
     // 1. s_hat[w] = ntt(s_w), canonical, and its gamma products.
-    tile_in<P>(slab,
-               [&](int pp) -> const int16_t * {
-                 return pp < nvalid ? s + ((i0 + pp) * K + w) * kN : nullptr;
-               },
-               lane);
+    if constexpr (SBITS == 0) {
+      tile_in<P>(slab,
+                 [&](int pp) -> const int16_t * {
+                   return pp < nvalid ? static_cast<const int16_t *>(s) + ((i0 + pp) * K + w) * kN
+                                      : nullptr;
+                 },
+                 lane);
+    } else {
+      constexpr size_t PPB = packed_poly_bytes<SBITS>();
+      tile_in_packed<P, SBITS>(slab,
+                               [&](int pp) -> const uint8_t * {
+                                 return pp < nvalid ? static_cast<const uint8_t *>(s) +
+                                                          ((i0 + pp) * K + w) * PPB
+                                                    : nullptr;
+                               },
+                               lane);
+    }
     wave_sync();
     pass_m16<kFormInter>(slab + tt * kStride, g, f1lo, f1hi, fb1);
     wave_sync();
@@ -192,7 +208,7 @@ __global__ __launch_bounds__(64 * K) void k_matvec(int16_t *__restrict__ t,
 
 }  // namespace
 
-int launch_matvec(int k, int16_t *t, const int16_t *A, const int16_t *s, size_t ninst,
+int launch_matvec(int k, int s_bits, int16_t *t, const int16_t *A, const void *s, size_t ninst,
                   const DeviceTables *tab, int ncu, hipStream_t stream)
 {
   const size_t ntiles = (ninst + kTileInst - 1) / kTileInst;
@@ -202,10 +218,20 @@ int launch_matvec(int k, int16_t *t, const int16_t *A, const int16_t *s, size_t 
     hipLaunchKernelGGL(kernel, grid, dim3(64 * waves), 0, stream, t, A, s, ninst, tab);
     return 0;
   };
-  switch (k) {
-    case 2: return go(k_matvec<2>, 2);
-    case 3: return go(k_matvec<3>, 3);
-    case 4: return go(k_matvec<4>, 4);
+  auto with_k = [&](auto kbits) {
+    constexpr int B = decltype(kbits)::value;
+    switch (k) {
+      case 2: return go(k_matvec<2, B>, 2);
+      case 3: return go(k_matvec<3, B>, 3);
+      case 4: return go(k_matvec<4, B>, 4);
+      default: return -1;
+    }
+  };
+  switch (s_bits) {
+    case 0: return with_k(std::integral_constant<int, 0>{});
+    case 2: return with_k(std::integral_constant<int, 2>{});
+    case 3: return with_k(std::integral_constant<int, 3>{});
+    case 4: return with_k(std::integral_constant<int, 4>{});
     default: return -1;
   }
 }

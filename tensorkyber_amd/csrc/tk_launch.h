@@ -25,6 +25,16 @@ struct RaggedTiles {
   size_t npoly;
 };
 
+// A contiguous batch of small-coefficient polys in the packed BITS-bit form
+// (see packed_poly_bytes): poly p at in + p * packed bytes, its forward
+// transform written to out + p*256.
+template <int BITS>
+struct PackedTiles {
+  const uint8_t *in;
+  int16_t *out;
+  size_t npoly;
+};
+
 // Grid shape of the tile kernels: waves per workgroup, and the grid as a
 // multiple of the workgroups resident at once (from the occupancy
 // calculator).  Each wave strides over tiles; a factor above 1 lets the
@@ -49,7 +59,17 @@ int launch_mfma(int shape, bool inverse, int form, int16_t *out, const int16_t *
 int launch_mfma_ragged(int shape, bool inverse, int form, const RaggedTiles &io,
                        const DeviceTables *tab, const LaunchShape &ls, hipStream_t stream);
 
-int launch_matvec(int k, int16_t *t, const int16_t *A, const int16_t *s, size_t ninst,
+// Forward transform of packed small-coefficient polys (bits 2, 3 or 4).
+int launch_mfma_packed(int shape, int bits, int form, int16_t *out, const uint8_t *in,
+                       size_t npoly, const DeviceTables *tab, const LaunchShape &ls,
+                       hipStream_t stream);
+
+// Packs int16 polys into the BITS-bit form (bits 2, 3 or 4).
+int launch_pack_small(int bits, uint8_t *out, const int16_t *in, size_t npoly, int ncu,
+                      hipStream_t stream);
+
+// s_bits = 0: s is int16 polys; 2, 3 or 4: s is packed.
+int launch_matvec(int k, int s_bits, int16_t *t, const int16_t *A, const void *s, size_t ninst,
                   const DeviceTables *tab, int ncu, hipStream_t stream);
 
 int launch_basemul(int form, int16_t *r, const int16_t *a, const int16_t *b, size_t npoly,

@@ -103,6 +103,81 @@ __device__ __forceinline__ int16_t *tile_dst(const RaggedTiles &, size_t, int pp
   return lds[pp];
 }
 
+// Packed tiles: output addresses as for linear tiles; the input is staged
+// from the packed buffer (stage_in below).
+template <int P, int BITS>
+__device__ __forceinline__ int16_t *tile_fetch(const PackedTiles<BITS> &, size_t, int)
+{
+  return nullptr;
+}
+
+template <int P, int BITS>
+__device__ __forceinline__ void tile_prepare(const PackedTiles<BITS> &io, size_t tile, int lane,
+                                             int16_t *, int16_t **lds)
+{
+  if (lane < P) {
+    const size_t p = tile * P + lane;
+    lds[P + lane] = p < io.npoly ? io.out + p * kN : nullptr;
+  }
+  wave_sync();
+}
+
+template <int P, int BITS>
+__device__ __forceinline__ int16_t *tile_dst(const PackedTiles<BITS> &, size_t, int pp,
+                                             int16_t *const *lds)
+{
+  return lds[P + pp];
+}
+
+// Staging registers: packed tiles are small (4-16 dwords per lane), so the
+// next tile's loads are issued into registers before the current tile's
+// stores.  The wait for those loads then does not include the stores'
+// completion (vmcnt counts both, in issue order).  An int16 tile (8 KB per
+// wave) held that way costs a wave of occupancy, so it is staged directly.
+template <int P, class Tiles>
+struct StageRegs {
+};
+
+template <int P, int BITS>
+struct StageRegs<P, PackedTiles<BITS>> {
+  uint32_t v[packed_regs<P, BITS>()];
+};
+
+template <int P, class Tiles>
+__device__ __forceinline__ void stage_load(const Tiles &, size_t, StageRegs<P, Tiles> &, int)
+{
+}
+
+template <int P, int BITS>
+__device__ __forceinline__ void stage_load(const PackedTiles<BITS> &io, size_t tile,
+                                           StageRegs<P, PackedTiles<BITS>> &r, int lane)
+{
+  constexpr size_t PPB = packed_poly_bytes<BITS>();
+  packed_load<P, BITS>(r.v,
+                       [&](int pp) -> const uint8_t * {
+                         const size_t p = tile * P + pp;
+                         return p < io.npoly ? io.in + p * PPB : nullptr;
+                       },
+                       lane);
+}
+
+// Stage tile `tile` into the slab L: int16 polys through their parked
+// addresses, packed polys from the staging registers.
+template <int P, class Tiles>
+__device__ __forceinline__ void stage_in(const Tiles &io, size_t tile, int16_t *L,
+                                         int16_t *const *lds, const StageRegs<P, Tiles> &, int lane)
+{
+  tile_in<P>(L, [&](int pp) { return tile_src<P>(io, tile, pp, lds); }, lane);
+}
+
+template <int P, int BITS>
+__device__ __forceinline__ void stage_in(const PackedTiles<BITS> &, size_t, int16_t *L,
+                                         int16_t *const *, const StageRegs<P, PackedTiles<BITS>> &r,
+                                         int lane)
+{
+  packed_park<P, BITS>(L, r.v, lane);
+}
+
 template <int FORM, bool INV, int WAVES, class Tiles>
 __global__ __launch_bounds__(64 * WAVES) void k_mfma16(Tiles io,
                                                         const DeviceTables *__restrict__ tab)
@@ -144,10 +219,12 @@ __global__ __launch_bounds__(64 * WAVES) void k_mfma16(Tiles io,
   const size_t stride = (size_t)gridDim.x * WAVES;
   size_t tile = (size_t)blockIdx.x * WAVES + w;
   int16_t *pre = tile_fetch<P>(io, tile, lane);
+  StageRegs<P, Tiles> sr;
+  stage_load<P>(io, tile, sr, lane);
   for (; tile < ntiles; tile += stride) {
     tile_prepare<P>(io, tile, lane, pre, s_ptr[w]);
     pre = tile_fetch<P>(io, tile + stride, lane);
-    tile_in<P>(L, [&](int pp) { return tile_src<P>(io, tile, pp, s_ptr[w]); }, lane);
+    stage_in<P>(io, tile, L, s_ptr[w], sr, lane);
     wave_sync();
     if (!INV) {
       pass_m16<kFormInter>(Pp, g, a1lo, a1hi, bias1);
@@ -161,6 +238,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_mfma16(Tiles io,
       pass_m16<FORM>(Pp, g, a1lo, a1hi, bias1);
     }
     wave_sync();
+    stage_load<P>(io, tile + stride, sr, lane);
     tile_out<P>(L, [&](int pp) { return tile_dst<P>(io, tile, pp, s_ptr[w]); }, lane);
     wave_sync();
   }
@@ -202,10 +280,12 @@ __global__ __launch_bounds__(64 * WAVES) void k_mfma32(Tiles io,
   const size_t stride = (size_t)gridDim.x * WAVES;
   size_t tile = (size_t)blockIdx.x * WAVES + w;
   int16_t *pre = tile_fetch<P>(io, tile, lane);
+  StageRegs<P, Tiles> sr;
+  stage_load<P>(io, tile, sr, lane);
   for (; tile < ntiles; tile += stride) {
     tile_prepare<P>(io, tile, lane, pre, s_ptr[w]);
     pre = tile_fetch<P>(io, tile + stride, lane);
-    tile_in<P>(L, [&](int pp) { return tile_src<P>(io, tile, pp, s_ptr[w]); }, lane);
+    stage_in<P>(io, tile, L, s_ptr[w], sr, lane);
     wave_sync();
     if (!INV) {
       pass_m32<kFormInter>(Pp, h, a1, bias1a, bias1b);
@@ -217,6 +297,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_mfma32(Tiles io,
       pass_m32<FORM>(Pp, h, a1, bias1a, bias1b);
     }
     wave_sync();
+    stage_load<P>(io, tile + stride, sr, lane);
     tile_out<P>(L, [&](int pp) { return tile_dst<P>(io, tile, pp, s_ptr[w]); }, lane);
     wave_sync();
   }
@@ -284,6 +365,24 @@ int launch_mfma_ragged(int shape, bool inverse, int form, const RaggedTiles &io,
                        const DeviceTables *tab, const LaunchShape &ls, hipStream_t stream)
 {
   return launch_any(shape, inverse, form, io, tab, ls, stream);
+}
+
+int launch_mfma_packed(int shape, int bits, int form, int16_t *out, const uint8_t *in,
+                       size_t npoly, const DeviceTables *tab, const LaunchShape &ls,
+                       hipStream_t stream)
+{
+  if (ls.waves != 2 && ls.waves != 4)
+    return -1;
+  auto go = [&](auto io) {
+    return shape == 32 ? launch_form<32, false>(form, ls.waves, ls.ncu, ls.oversub, io, tab, stream)
+                       : launch_form<16, false>(form, ls.waves, ls.ncu, ls.oversub, io, tab, stream);
+  };
+  switch (bits) {
+    case 2: return go(PackedTiles<2>{in, out, npoly});
+    case 3: return go(PackedTiles<3>{in, out, npoly});
+    case 4: return go(PackedTiles<4>{in, out, npoly});
+    default: return -1;
+  }
 }
 
 }  // namespace tk

@@ -71,6 +71,85 @@ __device__ __forceinline__ void tile_out(const int16_t *L, Addr dst, int lane)
   }
 }
 
+// ---------------------------------------------------------------------------
+// Packed small-coefficient input, the format of the reference's pack_small
+// (ntt_bench.cu:749-757): BITS-bit two's-complement fields, 8/BITS fields per
+// byte, coefficient i of a poly in byte i/(8/BITS) at bit BITS*(i%(8/BITS)).
+// A poly takes 256/(8/BITS) bytes: 64 for BITS = 2, 128 for BITS = 3 or 4.
+// ---------------------------------------------------------------------------
+template <int BITS>
+constexpr int packed_poly_bytes()
+{
+  return kN / (8 / BITS);
+}
+
+// One packed dword (coefficients c0 .. c0 + 4*(8/BITS) - 1) -> int16 at dst
+// (16-byte aligned, contiguous coefficients), sign-extending each field.
+template <int BITS>
+__device__ __forceinline__ void unpack_dword(uint32_t raw, int16_t *dst)
+{
+  constexpr int PER = 8 / BITS;  // fields per byte
+  constexpr int CPD = 4 * PER;   // coefficients per dword: 16 or 8
+  uint32_t pr[CPD / 2];
+#pragma unroll
+  for (int j = 0; j < CPD / 2; j++) {
+    const int i0 = 2 * j, i1 = 2 * j + 1;
+    const int lo = __builtin_amdgcn_sbfe((int)raw, 8 * (i0 / PER) + BITS * (i0 % PER), BITS);
+    const int hi = __builtin_amdgcn_sbfe((int)raw, 8 * (i1 / PER) + BITS * (i1 % PER), BITS);
+    pr[j] = pack2((uint32_t)lo, (uint32_t)hi);
+  }
+#pragma unroll
+  for (int v = 0; v < CPD / 8; v++)
+    *reinterpret_cast<v4i *>(dst + 8 * v) =
+        v4i{(int)pr[4 * v], (int)pr[4 * v + 1], (int)pr[4 * v + 2], (int)pr[4 * v + 3]};
+}
+
+// Packed tile staging, split so the loads of the next tile can be in flight
+// during the current one: packed_load reads P polys of
+// packed_poly_bytes<BITS>() each, one dword per lane (consecutive lanes take
+// consecutive dwords: coalesced), into v[P * bytes / 256]; packed_park
+// unpacks them into the int16 slab (neighbouring lanes write adjacent 16 or
+// 32 bytes: free of LDS bank conflicts).  `src` yields the packed address of
+// tile poly pp (nullptr = beyond the batch: zeros).
+template <int P, int BITS>
+constexpr int packed_regs()
+{
+  return P * packed_poly_bytes<BITS>() / 4 / 64;
+}
+
+template <int P, int BITS, class Addr>
+__device__ __forceinline__ void packed_load(uint32_t *v, Addr src, int lane)
+{
+  constexpr int DPP = packed_poly_bytes<BITS>() / 4;  // dwords per poly
+  static_assert(P * DPP % 64 == 0, "whole dwords per lane");
+#pragma unroll
+  for (int it = 0; it < packed_regs<P, BITS>(); it++) {
+    const int q = it * 64 + lane, pp = q / DPP, wd = q % DPP;
+    const uint8_t *p = src(pp);
+    v[it] = p ? __builtin_nontemporal_load(reinterpret_cast<const uint32_t *>(p) + wd) : 0u;
+  }
+}
+
+template <int P, int BITS>
+__device__ __forceinline__ void packed_park(int16_t *L, const uint32_t *v, int lane)
+{
+  constexpr int DPP = packed_poly_bytes<BITS>() / 4;
+  constexpr int CPD = 4 * (8 / BITS);  // coefficients per dword
+#pragma unroll
+  for (int it = 0; it < packed_regs<P, BITS>(); it++) {
+    const int q = it * 64 + lane, pp = q / DPP, wd = q % DPP;
+    unpack_dword<BITS>(v[it], L + pp * kStride + wd * CPD);
+  }
+}
+
+template <int P, int BITS, class Addr>
+__device__ __forceinline__ void tile_in_packed(int16_t *L, Addr src, int lane)
+{
+  uint32_t v[packed_regs<P, BITS>()];
+  packed_load<P, BITS>(v, src, lane);
+  packed_park<P, BITS>(L, v, lane);
+}
+
 typedef unsigned short us2 __attribute__((ext_vector_type(2)));
 
 // Final form of two packed values in [0, 2q): kFormInter keeps them,

@@ -44,6 +44,14 @@ def gpu(tk):
     return torch.device("cuda:0")
 
 
+@pytest.fixture(scope="session")
+def digests():
+    """Full-size digests of the compiled reference (tests/golden/digests.json)."""
+    import json
+    with open(os.path.join(ROOT, "tests", "golden", "digests.json")) as f:
+        return json.load(f)
+
+
 def canon(a):
     return np.mod(np.asarray(a, dtype=np.int32), 3329).astype(np.int16)
 

@@ -18,6 +18,7 @@ Usage: python tests/golden/make_golden.py
 """
 from __future__ import annotations
 
+import hashlib
 import json
 import os
 import sys
@@ -71,9 +72,26 @@ def ragged_pool():
 
 
 def matvec_inputs(ninst: int = 65536, k: int = 3):
-    A = tk.wl_uniform(ninst * k * k * N, 5, 0, Q, stream=tk.WL_MAT).reshape(ninst, k, k, N)
-    s = tk.wl_cbd2(ninst * k * N, 5, stream=tk.WL_SEC).reshape(ninst, k, N)
-    return A, s
+    return tk.matvec_workload(ninst, k)
+
+
+def packed_inputs(bits: int, npoly: int = 1 << 20) -> np.ndarray:
+    return tk.packed_workload(bits, npoly)
+
+
+def packed_digests(ref: Reference) -> dict:
+    from oracle.oracle import Oracle
+    o = Oracle()
+    d = {}
+    for bits in (2, 3, 4):
+        x = packed_inputs(bits)
+        b = o.pack_small(x, bits)
+        assert np.array_equal(o.unpack_small(b, bits), x)
+        d[f"bits{bits}"] = {"npoly": int(x.shape[0]), "input": tk.canon_digest(x),
+                            "packed_sha256": hashlib.sha256(b.tobytes()).hexdigest(),
+                            "ntt": tk.canon_digest(ref.ntt(x))}
+        print(f"packed {bits}: {d[f'bits{bits}']}", flush=True)
+    return d
 
 
 def digests(ref: Reference) -> dict:
@@ -108,17 +126,20 @@ def digests(ref: Reference) -> dict:
     d["config5"] = {"ninst": int(s.shape[0]), "k": 3, "A": tk.canon_digest(A),
                     "s": tk.canon_digest(s), "t": tk.canon_digest(t)}
     print(f"config 5: {d['config5']}", flush=True)
+    d["packed"] = packed_digests(ref)
     return d
 
 
 def main():
     ref = Reference()
-    fx = fixtures(ref)
-    np.savez_compressed(os.path.join(HERE, "fixtures.npz"), **fx)
+    if "--digests-only" not in sys.argv:
+        fx = fixtures(ref)
+        np.savez_compressed(os.path.join(HERE, "fixtures.npz"), **fx)
+        print("wrote fixtures.npz")
     d = digests(ref)
     with open(os.path.join(HERE, "digests.json"), "w") as f:
         json.dump(d, f, indent=1, sort_keys=True)
-    print("wrote fixtures.npz and digests.json")
+    print("wrote digests.json")
 
 
 if __name__ == "__main__":

@@ -68,6 +68,28 @@ def test_ragged_and_fused_validation(tk):
     assert L.tk_ragged_plan_destroy(None) == tk.EINVAL
 
 
+def test_packed_validation(tk):
+    L = tk.lib()
+    assert [L.tk_packed_poly_bytes(b) for b in (1, 2, 3, 4, 5, 8)] == [0, 64, 128, 128, 0, 0]
+    buf = (ctypes.c_int16 * (N * 64 + 16))()
+    al = (ctypes.addressof(buf) + 15) & ~15
+    assert L.pack_small_batch(al, al + 4096, 1, 5, None) == tk.EINVAL       # bad width
+    assert L.pack_small_batch(None, al, 1, 2, None) == tk.EINVAL
+    assert L.pack_small_batch(al + 8, al + 4096, 1, 2, None) == tk.EINVAL   # misaligned
+    assert L.pack_small_batch(al, al + 32, 2, 2, None) == tk.EINVAL         # overlap
+    assert L.pack_small_batch(None, None, 0, 2, None) == tk.OK              # empty: no-op
+    assert L.ntt_batch_packed(al, al + 4096, 1, 7, 0, 0, None) == tk.EINVAL
+    assert L.ntt_batch_packed(al, al + 4096, 1, 2, 99, 0, None) == tk.EINVAL  # bad method
+    assert L.ntt_batch_packed(al, al + 4096, 1, 2, 0, 99, None) == tk.EINVAL  # bad form
+    assert L.ntt_batch_packed(al, al + 4096, 1, 2, tk.METHOD_BUTTERFLY, 0, None) == tk.ENOTSUP
+    assert L.ntt_batch_packed(al, al + 4096, 1, 2, 0, tk.FORM_REF, None) == tk.ENOTSUP
+    assert L.ntt_batch_packed(al, al + 256, 1, 2, 0, 0, None) == tk.EINVAL  # overlap
+    assert L.matvec_ntt_fused_packed(al, al, al, 3, 1, 5, None) == tk.EINVAL
+    assert L.matvec_ntt_fused_packed(al, al, al, 5, 1, 3, None) == tk.EINVAL
+    assert L.matvec_ntt_fused_packed(al, al + 20000, al, 2, 1, 3, None) == tk.EINVAL  # t over s
+    assert L.matvec_ntt_fused_packed(None, None, None, 3, 0, 3, None) == tk.OK
+
+
 def test_workload_determinism(tk):
     a = tk.wl_uniform(1 << 17, 42)
     b = tk.wl_uniform(1 << 17, 42)

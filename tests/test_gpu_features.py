@@ -2,8 +2,6 @@
 the fused NTT -> basemul -> iNTT matrix-vector product, against the oracle
 and the full-size digests of the compiled reference."""
 import ctypes
-import json
-import os
 
 import numpy as np
 import pytest
@@ -13,13 +11,7 @@ from conftest import canon, centred
 pytestmark = pytest.mark.gpu
 
 N, Q = 256, 3329
-HERE = os.path.dirname(os.path.abspath(__file__))
 
-
-@pytest.fixture(scope="module")
-def digests():
-    with open(os.path.join(HERE, "golden", "digests.json")) as f:
-        return json.load(f)
 
 
 def _dev(a, device):

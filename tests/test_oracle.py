@@ -8,7 +8,6 @@
   and the schoolbook product through basemul, all mod q;
 * full-size digests of every benchmark configuration.
 """
-import json
 import os
 
 import numpy as np
@@ -26,11 +25,6 @@ def fx():
     with np.load(os.path.join(GOLD, "fixtures.npz"), allow_pickle=False) as f:
         return {k: f[k] for k in f.files}
 
-
-@pytest.fixture(scope="module")
-def digests():
-    with open(os.path.join(GOLD, "digests.json")) as f:
-        return json.load(f)
 
 
 def brv7(i):
