@@ -307,3 +307,125 @@ void oracle_unpack_small(int16_t *polys, const uint8_t *packed, size_t npoly, in
     }
   }
 }
+
+/* ------------------------------------------------------------------------ */
+/* K-PKE arithmetic cores (pqcrystals-kyber ref/poly.c, ref/polyvec.c and
+ * ref/indcpa.c, upstream of ref/ and not vendored; restated from the
+ * published reference: poly_tomont's factor is 2^32 mod q = 1353). */
+
+static void o_poly_reduce(int16_t r[ORACLE_N])
+{
+  for (int i = 0; i < ORACLE_N; i++)
+    r[i] = oracle_barrett_reduce(r[i]);
+}
+
+static void o_poly_tomont(int16_t r[ORACLE_N])
+{
+  const int16_t f = (int16_t)((1ULL << 32) % ORACLE_Q);
+  for (int i = 0; i < ORACLE_N; i++)
+    r[i] = oracle_montgomery_reduce((int32_t)r[i] * f);
+}
+
+static void o_poly_ntt(int16_t r[ORACLE_N])
+{
+  oracle_ntt(r);
+  o_poly_reduce(r);
+}
+
+/* polyvec_basemul_acc_montgomery: a and b are k consecutive polys. */
+static void o_basemul_acc(int16_t r[ORACLE_N], const int16_t *a, const int16_t *b, int k)
+{
+  int16_t t[ORACLE_N];
+  oracle_poly_basemul(r, a, b);
+  for (int j = 1; j < k; j++) {
+    oracle_poly_basemul(t, a + (size_t)j * ORACLE_N, b + (size_t)j * ORACLE_N);
+    for (int c = 0; c < ORACLE_N; c++)
+      r[c] = (int16_t)(r[c] + t[c]);
+  }
+  o_poly_reduce(r);
+}
+
+void oracle_kpke_keygen_batch(int16_t *t_hat, int16_t *s_hat, const int16_t *A_hat,
+                              const int16_t *s, const int16_t *e, int k, size_t ninst,
+                              int nthreads)
+{
+  oracle_zetas();
+  const long long n = (long long)ninst;
+#pragma omp parallel for schedule(static) num_threads(resolve_threads(nthreads))
+  for (long long inst = 0; inst < n; inst++) {
+    int16_t eh[4][ORACLE_N];
+    int16_t *sh = s_hat + (size_t)inst * k * ORACLE_N;
+    memcpy(sh, s + (size_t)inst * k * ORACLE_N, (size_t)k * ORACLE_N * sizeof(int16_t));
+    for (int j = 0; j < k; j++) {
+      o_poly_ntt(sh + (size_t)j * ORACLE_N);
+      memcpy(eh[j], e + ((size_t)inst * k + j) * ORACLE_N, sizeof(eh[j]));
+      o_poly_ntt(eh[j]);
+    }
+    for (int i = 0; i < k; i++) {
+      int16_t *pk = t_hat + ((size_t)inst * k + i) * ORACLE_N;
+      o_basemul_acc(pk, A_hat + ((size_t)inst * k + i) * k * ORACLE_N, sh, k);
+      o_poly_tomont(pk);
+      for (int c = 0; c < ORACLE_N; c++)
+        pk[c] = (int16_t)(pk[c] + eh[i][c]);
+      o_poly_reduce(pk);
+    }
+  }
+}
+
+void oracle_kpke_encrypt_batch(int16_t *u, int16_t *v, const int16_t *At_hat,
+                               const int16_t *t_hat, const int16_t *r, const int16_t *e1,
+                               const int16_t *e2, const int16_t *m, int k, size_t ninst,
+                               int nthreads)
+{
+  oracle_zetas();
+  const long long n = (long long)ninst;
+#pragma omp parallel for schedule(static) num_threads(resolve_threads(nthreads))
+  for (long long inst = 0; inst < n; inst++) {
+    int16_t rh[4 * ORACLE_N];
+    memcpy(rh, r + (size_t)inst * k * ORACLE_N, (size_t)k * ORACLE_N * sizeof(int16_t));
+    for (int j = 0; j < k; j++)
+      o_poly_ntt(rh + (size_t)j * ORACLE_N);
+    for (int i = 0; i < k; i++) {
+      int16_t *b = u + ((size_t)inst * k + i) * ORACLE_N;
+      const int16_t *ei = e1 + ((size_t)inst * k + i) * ORACLE_N;
+      o_basemul_acc(b, At_hat + ((size_t)inst * k + i) * k * ORACLE_N, rh, k);
+      oracle_invntt(b);
+      for (int c = 0; c < ORACLE_N; c++)
+        b[c] = (int16_t)(b[c] + ei[c]);
+      o_poly_reduce(b);
+    }
+    int16_t *vv = v + (size_t)inst * ORACLE_N;
+    const int16_t *e2i = e2 + (size_t)inst * ORACLE_N;
+    o_basemul_acc(vv, t_hat + (size_t)inst * k * ORACLE_N, rh, k);
+    oracle_invntt(vv);
+    for (int c = 0; c < ORACLE_N; c++)
+      vv[c] = (int16_t)(vv[c] + e2i[c]);
+    if (m) {
+      const int16_t *mi = m + (size_t)inst * ORACLE_N;
+      for (int c = 0; c < ORACLE_N; c++)
+        vv[c] = (int16_t)(vv[c] + mi[c]);
+    }
+    o_poly_reduce(vv);
+  }
+}
+
+void oracle_kpke_decrypt_batch(int16_t *w, const int16_t *s_hat, const int16_t *u,
+                               const int16_t *v, int k, size_t ninst, int nthreads)
+{
+  oracle_zetas();
+  const long long n = (long long)ninst;
+#pragma omp parallel for schedule(static) num_threads(resolve_threads(nthreads))
+  for (long long inst = 0; inst < n; inst++) {
+    int16_t bh[4 * ORACLE_N];
+    memcpy(bh, u + (size_t)inst * k * ORACLE_N, (size_t)k * ORACLE_N * sizeof(int16_t));
+    for (int j = 0; j < k; j++)
+      o_poly_ntt(bh + (size_t)j * ORACLE_N);
+    int16_t *mp = w + (size_t)inst * ORACLE_N;
+    const int16_t *vi = v + (size_t)inst * ORACLE_N;
+    o_basemul_acc(mp, s_hat + (size_t)inst * k * ORACLE_N, bh, k);
+    oracle_invntt(mp);
+    for (int c = 0; c < ORACLE_N; c++)
+      mp[c] = (int16_t)(vi[c] - mp[c]);
+    o_poly_reduce(mp);
+  }
+}

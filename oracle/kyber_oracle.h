@@ -20,6 +20,7 @@
  *   ntt_bench/ref/ntt.c:80-95     ntt                oracle_ntt
  *   ntt_bench/ref/ntt.c:106-126   invntt             oracle_invntt
  *   ntt_bench/ref/ntt.c:139-146   basemul            oracle_basemul
+ *   pqcrystals-kyber ref/indcpa.c  (upstream)        oracle_kpke_*_batch
  *   ntt_bench/ntt_bench.cu:749-757 pack_small        oracle_pack_small
  *   ntt_bench/ntt_bench.cu:325-348 pack_unpack       oracle_unpack_small
  *
@@ -75,6 +76,29 @@ void oracle_roundtrip_batch(int16_t *polys, size_t npoly, int nthreads);
  * reduce of every coefficient, then invntt. */
 void oracle_matvec_batch(int16_t *t, const int16_t *A_hat, const int16_t *s, int k,
                          size_t ninst, int nthreads);
+
+/* K-PKE arithmetic cores of ML-KEM, composed as the pqcrystals-kyber
+ * reference composes them in ref/indcpa.c (upstream of ref/, not vendored):
+ * poly_ntt = ntt then poly_reduce; polyvec_basemul_acc_montgomery = basemul
+ * of column 0, int16 adds of the others, poly_reduce; poly_tomont = fqmul by
+ * 2^32 mod q; poly_reduce = barrett_reduce of every coefficient; poly_add /
+ * poly_sub = int16 adds.  Every output ends in poly_reduce (centred).
+ *   keygen:  s_hat = poly_ntt(s); t_hat_i = reduce(tomont(A_hat_i . s_hat) +
+ *            poly_ntt(e_i))                       (indcpa_keypair_derand)
+ *   encrypt: r_hat = poly_ntt(r); u_i = reduce(invntt(At_hat_i . r_hat) +
+ *            e1_i); v = reduce(invntt(t_hat . r_hat) + e2 + m)  (indcpa_enc)
+ *   decrypt: w = reduce(v - invntt(s_hat . poly_ntt(u)))          (indcpa_dec)
+ * Shapes: matrices [ninst][k][k][256] (row i = polys i*k .. i*k+k-1),
+ * vectors [ninst][k][256], single polys [ninst][256].  m may be NULL. */
+void oracle_kpke_keygen_batch(int16_t *t_hat, int16_t *s_hat, const int16_t *A_hat,
+                              const int16_t *s, const int16_t *e, int k, size_t ninst,
+                              int nthreads);
+void oracle_kpke_encrypt_batch(int16_t *u, int16_t *v, const int16_t *At_hat,
+                               const int16_t *t_hat, const int16_t *r, const int16_t *e1,
+                               const int16_t *e2, const int16_t *m, int k, size_t ninst,
+                               int nthreads);
+void oracle_kpke_decrypt_batch(int16_t *w, const int16_t *s_hat, const int16_t *u,
+                               const int16_t *v, int k, size_t ninst, int nthreads);
 
 /* Packed small-coefficient polys (the TC-PACK input format): bits-bit
  * two's-complement fields, per = 8/bits per byte, coefficient i in byte

@@ -64,6 +64,47 @@ class _Batch:
         getattr(self.lib, p + "barrett_reduce").argtypes = [ctypes.c_int16]
         getattr(self.lib, p + "barrett_reduce").restype = ctypes.c_int16
         getattr(self.lib, p + "max_threads").restype = ctypes.c_int
+        sz, i = ctypes.c_size_t, ctypes.c_int
+        getattr(self.lib, p + "kpke_keygen_batch").argtypes = [_i16p] * 5 + [i, sz, i]
+        getattr(self.lib, p + "kpke_encrypt_batch").argtypes = [_i16p] * 7 + [ctypes.c_void_p,
+                                                                            i, sz, i]
+        getattr(self.lib, p + "kpke_decrypt_batch").argtypes = [_i16p] * 4 + [i, sz, i]
+
+    # K-PKE arithmetic cores (pqcrystals ref/indcpa.c composition); inputs are
+    # reshaped to [ninst, k(, k), 256], outputs returned the same way.
+    def kpke_keygen(self, A_hat, s, e, k: int, nthreads: int = 0):
+        s = np.ascontiguousarray(s, dtype=np.int16).reshape(-1, k, N)
+        ninst = s.shape[0]
+        A = np.ascontiguousarray(A_hat, dtype=np.int16).reshape(ninst, k, k, N)
+        e = np.ascontiguousarray(e, dtype=np.int16).reshape(ninst, k, N)
+        t_hat, s_hat = np.zeros_like(s), np.zeros_like(s)
+        getattr(self.lib, self._prefix + "kpke_keygen_batch")(
+            _ptr(t_hat), _ptr(s_hat), _ptr(A), _ptr(s), _ptr(e), k, ninst, nthreads)
+        return t_hat, s_hat
+
+    def kpke_encrypt(self, At_hat, t_hat, r, e1, e2, m, k: int, nthreads: int = 0):
+        r = np.ascontiguousarray(r, dtype=np.int16).reshape(-1, k, N)
+        ninst = r.shape[0]
+        At = np.ascontiguousarray(At_hat, dtype=np.int16).reshape(ninst, k, k, N)
+        t = np.ascontiguousarray(t_hat, dtype=np.int16).reshape(ninst, k, N)
+        e1 = np.ascontiguousarray(e1, dtype=np.int16).reshape(ninst, k, N)
+        e2 = np.ascontiguousarray(e2, dtype=np.int16).reshape(ninst, N)
+        mp = None if m is None else np.ascontiguousarray(m, dtype=np.int16).reshape(ninst, N)
+        u, v = np.zeros_like(r), np.zeros((ninst, N), dtype=np.int16)
+        getattr(self.lib, self._prefix + "kpke_encrypt_batch")(
+            _ptr(u), _ptr(v), _ptr(At), _ptr(t), _ptr(r), _ptr(e1), _ptr(e2),
+            None if mp is None else mp.ctypes.data, k, ninst, nthreads)
+        return u, v
+
+    def kpke_decrypt(self, s_hat, u, v, k: int, nthreads: int = 0):
+        u = np.ascontiguousarray(u, dtype=np.int16).reshape(-1, k, N)
+        ninst = u.shape[0]
+        sh = np.ascontiguousarray(s_hat, dtype=np.int16).reshape(ninst, k, N)
+        v = np.ascontiguousarray(v, dtype=np.int16).reshape(ninst, N)
+        w = np.zeros((ninst, N), dtype=np.int16)
+        getattr(self.lib, self._prefix + "kpke_decrypt_batch")(
+            _ptr(w), _ptr(sh), _ptr(u), _ptr(v), k, ninst, nthreads)
+        return w
 
     def _call(self, name, polys, nthreads):
         a = np.ascontiguousarray(polys, dtype=np.int16).copy()

@@ -52,6 +52,16 @@ def digests():
         return json.load(f)
 
 
+def negacyclic(a, b):
+    """a * b in Z_q[x]/(x^256 + 1), schoolbook (int64), canonical."""
+    a = np.asarray(a, dtype=np.int64) % 3329
+    b = np.asarray(b, dtype=np.int64) % 3329
+    full = np.convolve(a, b)
+    r = full[:256].copy()
+    r[: len(full) - 256] -= full[256:]
+    return r % 3329
+
+
 def canon(a):
     return np.mod(np.asarray(a, dtype=np.int32), 3329).astype(np.int16)
 

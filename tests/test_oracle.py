@@ -13,7 +13,7 @@ import os
 import numpy as np
 import pytest
 
-from conftest import canon
+from conftest import canon, negacyclic
 
 N, Q = 256, 3329
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -24,7 +24,6 @@ GOLD = os.path.join(HERE, "golden")
 def fx():
     with np.load(os.path.join(GOLD, "fixtures.npz"), allow_pickle=False) as f:
         return {k: f[k] for k in f.files}
-
 
 
 def brv7(i):
@@ -41,15 +40,6 @@ def ntt_closed_form(f):
         out[2 * p] = int(np.dot(f[0::2], pw) % Q)
         out[2 * p + 1] = int(np.dot(f[1::2], pw) % Q)
     return out
-
-
-def negacyclic(a, b):
-    a = np.asarray(a, dtype=np.int64) % Q
-    b = np.asarray(b, dtype=np.int64) % Q
-    full = np.convolve(a, b)
-    r = full[:N].copy()
-    r[: len(full) - N] -= full[N:]
-    return r % Q
 
 
 @pytest.mark.parametrize("cat", ["uniform", "full", "edge", "small"])
