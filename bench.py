@@ -301,6 +301,45 @@ def bench_packed(tk, torch, steps, npoly=1 << 20):
     return out
 
 
+def bench_kpke(tk, torch, steps, ninst=65536, k=3):
+    """K-PKE arithmetic cores (keygen, encryption, decryption), one fused
+    kernel each, on 65,536 rank-3 instances (kpke_workload, seed 9)."""
+    w = tk.kpke_workload(ninst, k)
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to("cuda:0")  # noqa: E731
+    A = dev(w["A_hat"])
+    At = dev(w["A_hat"].transpose(0, 2, 1, 3))
+    s, e, r, e1, e2, m = (dev(w[n]) for n in ("s", "e", "r", "e1", "e2", "m"))
+    t_hat, s_hat = torch.empty_like(s), torch.empty_like(s)
+    u, v = torch.empty_like(r), torch.empty_like(e2)
+    wout = torch.empty_like(e2)
+    stream = torch.cuda.current_stream()
+
+    def per_launch(fn):
+        warm(torch, fn)
+        e0, e1_ = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(steps):
+            fn()
+        e1_.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1_) / steps
+
+    polys = {"keygen": k * k + 4 * k, "encrypt": k * k + 4 * k + 3, "decrypt": 2 * k + 2}
+    fns = {
+        "keygen": lambda: tk.kpke_keygen(A, s, e, k, t_hat=t_hat, s_hat=s_hat),
+        "encrypt": lambda: tk.kpke_encrypt(At, t_hat, r, e1, e2, m, k, u=u, v=v),
+        "decrypt": lambda: tk.kpke_decrypt(s_hat, u, v, k, w=wout),
+    }
+    out = {"workload": f"{ninst} instances, k = {k}, A_hat uniform [0,q), CBD eta=2 noise, "
+                       "random message bits, seed 9"}
+    for name, fn in fns.items():
+        ms = per_launch(fn)
+        nbytes = polys[name] * 512
+        out[name] = {"us": ms * 1e3, "inst_s": ninst / (ms / 1e3), "bytes_per_inst": nbytes,
+                     "roofline_frac": ninst * nbytes / (ms / 1e3) / 1e9 / HBM_PEAK_GBS}
+    return out
+
+
 def check_parity(tk, torch, method, x, buf_fwd, buf_rt):
     """Canonical equality of the GPU forward and inverse against the oracle."""
     from oracle.oracle import Oracle
@@ -389,6 +428,8 @@ def main():
         extras["config5_matvec"] = bench_matvec(tk, torch, args.steps)
         print("[bench] packed input", file=sys.stderr, flush=True)
         extras["packed_input"] = bench_packed(tk, torch, args.steps)
+        print("[bench] K-PKE cores", file=sys.stderr, flush=True)
+        extras["kpke_cores"] = bench_kpke(tk, torch, args.steps)
 
     cpu = None
     if not args.no_cpu:

@@ -139,6 +139,44 @@ int tk_ragged_plan_destroy(tk_ragged_plan *plan);
 int matvec_ntt_fused(int16_t *t, const int16_t *A_hat, const int16_t *s, int k, size_t ninst,
                      void *stream);
 
+/* ---- K-PKE arithmetic cores -------------------------------------------- */
+
+/* The arithmetic of ML-KEM's K-PKE around the NTT path, one fused kernel
+ * each, composed as the pqcrystals-kyber reference composes it (ref/indcpa.c,
+ * ref/polyvec.c, ref/poly.c; upstream of ntt_bench/ref/, which vendors only
+ * its ntt.c and reduce.c).  Every output ends in that reference's
+ * poly_reduce (barrett_reduce, ref/reduce.c:35-42) and equals its raw int16
+ * values: centred representatives in [-(q-1)/2, (q-1)/2].  Sampling,
+ * hashing, matrix expansion and (de)compression stay with the caller.
+ * Shapes per instance: matrices [k][k][256] (row i = polys i*k .. i*k+k-1),
+ * vectors [k][256], single polys [256]; instances back to back; k in 2..4;
+ * device pointers, 16-byte aligned.  An output may alias exactly (same
+ * address) only the input named beside it; other overlaps are TK_EINVAL. */
+
+/* Key generation (indcpa_keypair_derand): s_hat = poly_ntt(s) (may alias s);
+ * t_hat_i = reduce(tomont(sum_j A_hat[i][j] o s_hat_j) + poly_ntt(e_i)) (may
+ * alias e), both in the NTT domain. */
+int kpke_keygen_core(int16_t *t_hat, int16_t *s_hat, const int16_t *A_hat, const int16_t *s,
+                     const int16_t *e, int k, size_t ninst, void *stream);
+
+/* Encryption (indcpa_enc), with At_hat the transposed matrix as gen_at
+ * yields it: r_hat = poly_ntt(r); u_i = reduce(invntt(sum_j At_hat[i][j] o
+ * r_hat_j) + e1_i) (may alias e1); v = reduce(invntt(sum_j t_hat_j o r_hat_j)
+ * + e2 + m) (may alias e2).  m is the message poly (poly_frommsg) or NULL.
+ * The adds are exact mod q: equal to the reference's int16 poly_add wherever
+ * that does not overflow, which holds for every ML-KEM input (noise of at
+ * most eta = 3, message coefficients 0 or (q+1)/2). */
+int kpke_encrypt_core(int16_t *u, int16_t *v, const int16_t *At_hat, const int16_t *t_hat,
+                      const int16_t *r, const int16_t *e1, const int16_t *e2, const int16_t *m,
+                      int k, size_t ninst, void *stream);
+
+/* Decryption (indcpa_dec): w = reduce(v - invntt(sum_j s_hat_j o
+ * poly_ntt(u_j))) (may alias v).  The subtraction is exact mod q: equal
+ * to the reference's int16 poly_sub wherever that does not overflow (always
+ * for decompressed v in [0, q)). */
+int kpke_decrypt_core(int16_t *w, const int16_t *s_hat, const int16_t *u, const int16_t *v, int k,
+                      size_t ninst, void *stream);
+
 /* ---- packed small-coefficient input ------------------------------------ */
 
 /* The packed form of the reference's TC-PACK path (ntt_bench.cu:278-391,

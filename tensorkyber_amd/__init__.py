@@ -93,6 +93,9 @@ def _declare(lib):
         "pack_small_batch": ([vp, vp, sz, i, vp], i),
         "ntt_batch_packed": ([vp, vp, sz, i, i, i, vp], i),
         "matvec_ntt_fused_packed": ([vp, vp, vp, i, sz, i, vp], i),
+        "kpke_keygen_core": ([vp, vp, vp, vp, vp, i, sz, vp], i),
+        "kpke_encrypt_core": ([vp, vp, vp, vp, vp, vp, vp, vp, i, sz, vp], i),
+        "kpke_decrypt_core": ([vp, vp, vp, vp, i, sz, vp], i),
         "tk_ragged_plan_create": ([ctypes.POINTER(vp), vp, sz, i, i, vp], i),
         "tk_ragged_plan_execute": ([vp, vp], i),
         "tk_ragged_plan_launch": ([vp, vp], i),
@@ -274,6 +277,59 @@ def matvec_packed(A_hat, s_packed, k: int, bits: int, out=None, stream=None):
     return out
 
 
+def _ptr_or_none(t):
+    return None if t is None else t.data_ptr()
+
+
+def kpke_keygen(A_hat, s, e, k: int, t_hat=None, s_hat=None, stream=None):
+    """K-PKE key generation core: (t_hat, s_hat) in the NTT domain, centred."""
+    A_hat, s, e = _as_polys(A_hat, "A_hat"), _as_polys(s, "s"), _as_polys(e, "e")
+    ninst = s.numel() // (k * N)
+    if s.numel() != ninst * k * N or e.numel() != s.numel() or A_hat.numel() != s.numel() * k:
+        raise ValueError("A_hat, s and e shapes disagree")
+    torch = _torch()
+    t_hat = torch.empty_like(s) if t_hat is None else _as_polys(t_hat, "t_hat")
+    s_hat = torch.empty_like(s) if s_hat is None else _as_polys(s_hat, "s_hat")
+    _check(lib().kpke_keygen_core(t_hat.data_ptr(), s_hat.data_ptr(), A_hat.data_ptr(),
+                                  s.data_ptr(), e.data_ptr(), k, ninst, _stream_ptr(stream)),
+           "kpke_keygen_core")
+    return t_hat, s_hat
+
+
+def kpke_encrypt(At_hat, t_hat, r, e1, e2, m, k: int, u=None, v=None, stream=None):
+    """K-PKE encryption core: (u [ninst*k, 256], v [ninst, 256]), centred."""
+    At_hat, t_hat, r = _as_polys(At_hat, "At_hat"), _as_polys(t_hat, "t_hat"), _as_polys(r, "r")
+    e1, e2 = _as_polys(e1, "e1"), _as_polys(e2, "e2")
+    ninst = r.numel() // (k * N)
+    if (r.numel() != ninst * k * N or t_hat.numel() != r.numel() or e1.numel() != r.numel()
+            or At_hat.numel() != r.numel() * k or e2.numel() != ninst * N
+            or (m is not None and _as_polys(m, "m").numel() != ninst * N)):
+        raise ValueError("K-PKE encryption shapes disagree")
+    torch = _torch()
+    u = torch.empty((ninst * k, N), dtype=torch.int16, device=r.device) if u is None \
+        else _as_polys(u, "u")
+    v = torch.empty((ninst, N), dtype=torch.int16, device=r.device) if v is None \
+        else _as_polys(v, "v")
+    _check(lib().kpke_encrypt_core(u.data_ptr(), v.data_ptr(), At_hat.data_ptr(), t_hat.data_ptr(),
+                                   r.data_ptr(), e1.data_ptr(), e2.data_ptr(), _ptr_or_none(m), k,
+                                   ninst, _stream_ptr(stream)), "kpke_encrypt_core")
+    return u, v
+
+
+def kpke_decrypt(s_hat, u, v, k: int, w=None, stream=None):
+    """K-PKE decryption core: w = reduce(v - invntt(s_hat . poly_ntt(u)))."""
+    s_hat, u, v = _as_polys(s_hat, "s_hat"), _as_polys(u, "u"), _as_polys(v, "v")
+    ninst = u.numel() // (k * N)
+    if u.numel() != ninst * k * N or s_hat.numel() != u.numel() or v.numel() != ninst * N:
+        raise ValueError("K-PKE decryption shapes disagree")
+    torch = _torch()
+    w = torch.empty((ninst, N), dtype=torch.int16, device=u.device) if w is None \
+        else _as_polys(w, "w")
+    _check(lib().kpke_decrypt_core(w.data_ptr(), s_hat.data_ptr(), u.data_ptr(), v.data_ptr(), k,
+                                   ninst, _stream_ptr(stream)), "kpke_decrypt_core")
+    return w
+
+
 # ---------------------------------------------------------------------------
 # host introspection and synthetic workloads (no device needed)
 # ---------------------------------------------------------------------------
@@ -363,6 +419,21 @@ def matvec_workload(ninst: int = 65536, k: int = 3):
     return A, s
 
 
+def kpke_workload(ninst: int = 65536, k: int = 3, seed: int = 9) -> dict:
+    """K-PKE inputs: A_hat uniform [0, q) (NTT domain, as matrix expansion
+    yields it); s, e, r, e1, e2 CBD eta = 2 on consecutive seeds of the secret
+    stream; m = (q+1)/2 times uniform message bits (poly_frommsg).  Arrays are
+    [ninst, k, k, 256], [ninst, k, 256] or [ninst, 256]; "bits" holds the
+    message bits."""
+    A = wl_uniform(ninst * k * k * N, seed, 0, Q, stream=WL_MAT).reshape(ninst, k, k, N)
+    vec = {name: wl_cbd2(ninst * k * N, seed + 1 + i).reshape(ninst, k, N)
+           for i, name in enumerate(("s", "e", "r", "e1"))}
+    e2 = wl_cbd2(ninst * N, seed + 5).reshape(ninst, N)
+    bits = wl_uniform(ninst * N, seed + 6, 0, 2).reshape(ninst, N)
+    m = (bits * ((Q + 1) // 2)).astype(np.int16)
+    return {"A_hat": A, **vec, "e2": e2, "m": m, "bits": bits}
+
+
 def packed_workload(bits: int, npoly: int = 1 << 20) -> np.ndarray:
     """Small-coefficient polys for the packed forward path: ternary (seed 6)
     for 2 bits, CBD eta = 2 (seed 7) for 3 bits, uniform [-8, 8) (seed 8) for
@@ -381,7 +452,9 @@ def packed_workload(bits: int, npoly: int = 1 << 20) -> np.ndarray:
 __all__ = [
     "N", "Q", "TkError", "load_library", "lib", "ntt", "invntt", "basemul", "matvec",
     "packed_poly_bytes", "pack_small", "ntt_packed", "matvec_packed",
+    "kpke_keygen", "kpke_encrypt", "kpke_decrypt",
     "host_tables", "host_mfma_operands", "workload", "matvec_workload", "packed_workload",
+    "kpke_workload",
     "wl_uniform", "wl_edge_mix", "wl_cbd2", "wl_ragged_layout", "canon_digest", "TkRequest",
     "METHODS", "FORMS",
 ]

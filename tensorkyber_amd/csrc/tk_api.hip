@@ -285,9 +285,138 @@ int matvec_ntt_fused(int16_t *t, const int16_t *A_hat, const int16_t *s, int k, 
   int rc = init_device(&st);
   if (rc)
     return rc;
-  if (launch_matvec(k, 0, t, A_hat, s, ninst, st->tables, st->ncu, (hipStream_t)stream))
+  FusedIo io{};
+  io.out0 = t;
+  io.mat = A_hat;
+  io.x = s;
+  io.ninst = ninst;
+  if (launch_fused(k, kFusedMatvec, 0, io, st->tables, st->ncu, (hipStream_t)stream))
     return TK_ENOTSUP;
   return after_launch();
+}
+
+namespace {
+
+// One buffer of a fused K-PKE call: its address, size and, for an output,
+// the one input it may alias exactly (the kernel reads that input's row in
+// the same lane before it writes the output row at the same address).
+struct Buf {
+  const void *p;
+  size_t bytes;
+  const void *may_alias;
+};
+
+bool bufs_ok(const Buf *out, int nout, const Buf *in, int nin)
+{
+  auto overlap = [](const Buf &a, const Buf &b) {
+    const uintptr_t x = (uintptr_t)a.p, y = (uintptr_t)b.p;
+    return a.p && b.p && x < y + b.bytes && y < x + a.bytes;
+  };
+  for (int i = 0; i < nout; i++) {
+    if (!aligned16(out[i].p))
+      return false;
+    for (int j = i + 1; j < nout; j++)
+      if (overlap(out[i], out[j]))
+        return false;
+    for (int j = 0; j < nin; j++)
+      if (overlap(out[i], in[j]) && !(out[i].may_alias == in[j].p && out[i].p == in[j].p))
+        return false;
+  }
+  for (int j = 0; j < nin; j++)
+    if (in[j].p && !aligned16(in[j].p))
+      return false;
+  return true;
+}
+
+int run_fused(int k, int mode, const FusedIo &io, void *stream)
+{
+  DevState *st = nullptr;
+  int rc = init_device(&st);
+  if (rc)
+    return rc;
+  if (launch_fused(k, mode, 0, io, st->tables, st->ncu, (hipStream_t)stream))
+    return TK_ENOTSUP;
+  return after_launch();
+}
+
+}  // namespace
+
+int kpke_keygen_core(int16_t *t_hat, int16_t *s_hat, const int16_t *A_hat, const int16_t *s,
+                     const int16_t *e, int k, size_t ninst, void *stream)
+{
+  if (k < 2 || k > 4)
+    return TK_EINVAL;
+  if (ninst == 0)
+    return TK_OK;
+  if (!t_hat || !s_hat || !A_hat || !s || !e || ninst > kMaxPolys / 16)
+    return TK_EINVAL;
+  const size_t vb = ninst * k * kN * sizeof(int16_t);
+  const Buf out[2] = {{t_hat, vb, e}, {s_hat, vb, s}};
+  const Buf in[3] = {{A_hat, vb * k, nullptr}, {s, vb, nullptr}, {e, vb, nullptr}};
+  if (!bufs_ok(out, 2, in, 3))
+    return TK_EINVAL;
+  FusedIo io{};
+  io.out0 = t_hat;
+  io.out1 = s_hat;
+  io.mat = A_hat;
+  io.x = s;
+  io.add0 = e;
+  io.ninst = ninst;
+  return run_fused(k, kFusedKeygen, io, stream);
+}
+
+int kpke_encrypt_core(int16_t *u, int16_t *v, const int16_t *At_hat, const int16_t *t_hat,
+                      const int16_t *r, const int16_t *e1, const int16_t *e2, const int16_t *m,
+                      int k, size_t ninst, void *stream)
+{
+  if (k < 2 || k > 4)
+    return TK_EINVAL;
+  if (ninst == 0)
+    return TK_OK;
+  if (!u || !v || !At_hat || !t_hat || !r || !e1 || !e2 || ninst > kMaxPolys / 16)
+    return TK_EINVAL;
+  const size_t pb = ninst * kN * sizeof(int16_t), vb = pb * k;
+  const Buf out[2] = {{u, vb, e1}, {v, pb, e2}};
+  const Buf in[6] = {{At_hat, vb * k, nullptr}, {t_hat, vb, nullptr}, {r, vb, nullptr},
+                     {e1, vb, nullptr},         {e2, pb, nullptr},    {m, pb, nullptr}};
+  if (!bufs_ok(out, 2, in, 6))
+    return TK_EINVAL;
+  if (m && m == v)  // v may alias e2 only: m is read after e2's row is added
+    return TK_EINVAL;
+  FusedIo io{};
+  io.out0 = u;
+  io.out1 = v;
+  io.mat = At_hat;
+  io.mat2 = t_hat;
+  io.x = r;
+  io.add0 = e1;
+  io.add1 = e2;
+  io.add2 = m;
+  io.ninst = ninst;
+  return run_fused(k, kFusedEncrypt, io, stream);
+}
+
+int kpke_decrypt_core(int16_t *w, const int16_t *s_hat, const int16_t *u, const int16_t *v, int k,
+                      size_t ninst, void *stream)
+{
+  if (k < 2 || k > 4)
+    return TK_EINVAL;
+  if (ninst == 0)
+    return TK_OK;
+  if (!w || !s_hat || !u || !v || ninst > kMaxPolys / 16)
+    return TK_EINVAL;
+  const size_t pb = ninst * kN * sizeof(int16_t), vb = pb * k;
+  const Buf out[1] = {{w, pb, v}};
+  const Buf in[3] = {{s_hat, vb, nullptr}, {u, vb, nullptr}, {v, pb, nullptr}};
+  if (!bufs_ok(out, 1, in, 3))
+    return TK_EINVAL;
+  FusedIo io{};
+  io.out0 = w;
+  io.mat = s_hat;
+  io.x = u;
+  io.add0 = v;
+  io.ninst = ninst;
+  return run_fused(k, kFusedDecrypt, io, stream);
 }
 
 size_t tk_packed_poly_bytes(int bits)
@@ -362,7 +491,12 @@ int matvec_ntt_fused_packed(int16_t *t, const int16_t *A_hat, const uint8_t *s_p
   int rc = init_device(&st);
   if (rc)
     return rc;
-  if (launch_matvec(k, bits, t, A_hat, s_packed, ninst, st->tables, st->ncu, (hipStream_t)stream))
+  FusedIo io{};
+  io.out0 = t;
+  io.mat = A_hat;
+  io.x = s_packed;
+  io.ninst = ninst;
+  if (launch_fused(k, kFusedMatvec, bits, io, st->tables, st->ncu, (hipStream_t)stream))
     return TK_ENOTSUP;
   return after_launch();
 }

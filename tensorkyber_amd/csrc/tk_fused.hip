@@ -14,6 +14,24 @@
 // HBM traffic per instance is (k*k + 2k) * 512 bytes; nothing intermediate
 // leaves the CU.
 //
+// The same kernel, by MODE, runs the K-PKE arithmetic cores of ML-KEM as the
+// pqcrystals-kyber reference composes them (ref/indcpa.c, upstream of
+// ref/), each output ending in that reference's poly_reduce (barrett_reduce,
+// ref/reduce.c:35-42), so outputs are its raw int16 values:
+//   keygen:  s_hat = poly_ntt(s) is written in step 1; step 3 stages e_i,
+//            forward-transforms it and writes t_hat_i = reduce(row_i +
+//            e_hat_i) in the NTT domain (no inverse);
+//   encrypt: k+1 waves and rows: u_i from row i of the transposed matrix,
+//            v from t_hat; step 3 inverse-transforms and adds e1_i, or e2 and
+//            m;
+//   decrypt: one row s_hat . u_hat; step 3 inverse-transforms and writes
+//            w = reduce(v - row).
+// The adds are exact mod q.  The reference's poly_add / poly_sub are int16
+// adds on invntt's (-q, q) representatives; they agree with these wherever
+// they do not overflow int16, which holds for every ML-KEM input (noise of
+// at most eta = 3, message coefficients 0 or (q+1)/2, decompressed u and v
+// in [0, q)).
+//
 // Arithmetic.  The products are accumulated in the plain domain:
 //   r0 = sum_j a0*b0 + a1*(gamma*b1),  r1 = sum_j a0*b1 + a1*b0   (mod q)
 // per degree-1 pair with gamma = zetas[64+i] * 2^-16 (and -gamma for the odd
@@ -63,22 +81,83 @@ __device__ __forceinline__ long ld_a(const uint32_t (*m)[64][2], int limb, int l
   return *reinterpret_cast<const long *>(m[limb][lane]);
 }
 
-// SBITS = 0: s holds int16 polys [ninst][K][256]; SBITS = 2, 3, 4: s holds
-// the packed small-coefficient form [ninst][K][packed bytes].
-template <int K, int SBITS>
-__global__ __launch_bounds__(64 * K) void k_matvec(int16_t *__restrict__ t,
-                                                   const int16_t *__restrict__ A,
-                                                   const void *__restrict__ s, size_t ninst,
-                                                   const DeviceTables *__restrict__ tab)
+// barrett_reduce of each int16 lane of a v4i (centred representatives).
+__device__ __forceinline__ v4i barrett_v4(const v4i x)
 {
-  constexpr int P = kTileInst;
-  __shared__ __align__(16) int16_t s_slab[K][P * kStride];
+  v4i r;
+#pragma unroll
+  for (int d = 0; d < 4; d++) {
+    const int16_t lo = (int16_t)(x[d] & 0xffff), hi = (int16_t)(x[d] >> 16);
+    r[d] = (int)pack2((uint16_t)barrett16(lo), (uint16_t)barrett16(hi));
+  }
+  return r;
+}
+
+// Lane-wise int16 add (no wrap for the operands it is used on: < 3q).  The
+// elements are copied to scalars first: __builtin_bit_cast of a vector
+// element lvalue reads element 0 here, whatever the index.
+__device__ __forceinline__ v4i add16(const v4i a, const v4i b)
+{
+  typedef short v2s __attribute__((ext_vector_type(2)));
+  v4i r;
+#pragma unroll
+  for (int d = 0; d < 4; d++) {
+    const int ad = a[d], bd = b[d];
+    r[d] = __builtin_bit_cast(int, __builtin_bit_cast(v2s, ad) + __builtin_bit_cast(v2s, bd));
+  }
+  return r;
+}
+
+// Centred representative of SX*x + y + z per int16 lane, exact mod q for any
+// int16 operands (|sum| < 21q is lifted into [0, 2^24) and reduced).
+template <int SX>
+__device__ __forceinline__ v4i centred_sum(const v4i x, const v4i y, const v4i z)
+{
+  auto one = [](int a) -> uint32_t {
+    uint32_t r = red24((uint32_t)(a + 21 * kQ));  // [0, 2q)
+    r = min(r, r - (uint32_t)kQ);                  // [0, q)
+    return r > (uint32_t)kCentre ? r - (uint32_t)kQ : r;
+  };
+  v4i out;
+#pragma unroll
+  for (int d = 0; d < 4; d++) {
+    const int xl = (int16_t)(x[d] & 0xffff), xh = x[d] >> 16;
+    const int yl = (int16_t)(y[d] & 0xffff), yh = y[d] >> 16;
+    const int zl = (int16_t)(z[d] & 0xffff), zh = z[d] >> 16;
+    out[d] = (int)pack2(one(SX * xl + yl + zl), one(SX * xh + yh + zh));
+  }
+  return out;
+}
+
+template <int MODE, int K>
+constexpr int fused_waves()
+{
+  return MODE == kFusedEncrypt ? K + 1 : K;
+}
+
+template <int MODE, int K>
+constexpr int fused_rows()
+{
+  return MODE == kFusedEncrypt ? K + 1 : (MODE == kFusedDecrypt ? 1 : K);
+}
+
+// K: module rank; MODE: kFusedMatvec / Keygen / Encrypt / Decrypt (FusedIo
+// documents the buffers of each); SBITS = 0: the transformed vector holds
+// int16 polys [ninst][K][256]; 2, 3, 4: the packed small-coefficient form
+// [ninst][K][packed bytes].
+template <int K, int MODE, int SBITS>
+__global__ __launch_bounds__((64 * fused_waves<MODE, K>())) void k_fused(
+    const FusedIo io, const DeviceTables *__restrict__ tab)
+{
+  constexpr int P = kTileInst, W = fused_waves<MODE, K>(), R = fused_rows<MODE, K>();
+  __shared__ __align__(16) int16_t s_slab[W][P * kStride];
   __shared__ __align__(16) uint32_t s_g[K][P][64];  // (gamma*b1, (q-gamma)*b3) per group
   __shared__ __align__(16) int32_t s_bf2[16][16];
   __shared__ __align__(16) int32_t s_bi2[16][16];
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, tt = lane & 15;
+  const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int lane = threadIdx.x & 63, g = lane >> 4, tt = lane & 15;
 
-  for (int i = threadIdx.x; i < 256; i += 64 * K) {
+  for (int i = threadIdx.x; i < 256; i += 64 * W) {
     s_bf2[i >> 4][i & 15] = tab->bias_fwd2[i >> 4][i & 15];
     s_bi2[i >> 4][i & 15] = tab->bias_inv2[i >> 4][i & 15];
   }
@@ -107,132 +186,228 @@ __global__ __launch_bounds__(64 * K) void k_matvec(int16_t *__restrict__ t,
   const uint32_t gam = tab->gamma[lane], gam_neg = (uint32_t)kQ - gam;
 
   int16_t *const slab = s_slab[w];
-  const size_t ntiles = (ninst + P - 1) / P;
+  int16_t *const pslab = slab + tt * kStride;
+  // Matrix row of output row w for instance inst (K consecutive polys).
+  auto row_base = [&](size_t inst) -> const int16_t * {
+    if constexpr (MODE == kFusedEncrypt)
+      return w < K ? io.mat + (inst * K + w) * (size_t)(K * kN) : io.mat2 + inst * K * kN;
+    else if constexpr (MODE == kFusedDecrypt)
+      return io.mat + inst * K * kN;
+    else
+      return io.mat + (inst * K + w) * (size_t)(K * kN);
+  };
+  // Forward transform of the staged slab to canonical values.
+  auto forward = [&]() {
+    wave_sync();
+    pass_m16<kFormInter>(pslab, g, f1lo, f1hi, fb1);
+    wave_sync();
+    pass_c16<kFormCanon>(pslab, g, f2lo, f2hi, s_bf2);
+    wave_sync();
+  };
+
+  const size_t ntiles = (io.ninst + P - 1) / P;
   for (size_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const size_t i0 = tile * P;
-    const int nvalid = (int)(ninst - i0 < (size_t)P ? ninst - i0 : P);
+    const int nvalid = (int)(io.ninst - i0 < (size_t)P ? io.ninst - i0 : P);
 
-    // 1. s_hat[w] = ntt(s_w), canonical, and its gamma products.
-    if constexpr (SBITS == 0) {
-      tile_in<P>(slab,
-                 [&](int pp) -> const int16_t * {
-                   return pp < nvalid ? static_cast<const int16_t *>(s) + ((i0 + pp) * K + w) * kN
-                                      : nullptr;
-                 },
-                 lane);
-    } else {
-      constexpr size_t PPB = packed_poly_bytes<SBITS>();
-      tile_in_packed<P, SBITS>(slab,
-                               [&](int pp) -> const uint8_t * {
-                                 return pp < nvalid ? static_cast<const uint8_t *>(s) +
-                                                          ((i0 + pp) * K + w) * PPB
-                                                    : nullptr;
-                               },
-                               lane);
-    }
-    wave_sync();
-    pass_m16<kFormInter>(slab + tt * kStride, g, f1lo, f1hi, fb1);
-    wave_sync();
-    pass_c16<kFormCanon>(slab + tt * kStride, g, f2lo, f2hi, s_bf2);
-    wave_sync();
+    // 1. x_hat[w] = ntt(x_w), canonical, and its gamma products.
+    if (w < K) {
+      if constexpr (SBITS == 0) {
+        tile_in<P>(slab,
+                   [&](int pp) -> const int16_t * {
+                     return pp < nvalid ? static_cast<const int16_t *>(io.x) + ((i0 + pp) * K + w) * kN
+                                        : nullptr;
+                   },
+                   lane);
+      } else {
+        constexpr size_t PPB = packed_poly_bytes<SBITS>();
+        tile_in_packed<P, SBITS>(slab,
+                                 [&](int pp) -> const uint8_t * {
+                                   return pp < nvalid ? static_cast<const uint8_t *>(io.x) +
+                                                            ((i0 + pp) * K + w) * PPB
+                                                      : nullptr;
+                                 },
+                                 lane);
+      }
+      forward();
 #pragma unroll 4
-    for (int pp = 0; pp < P; pp++) {
-      const uint2 d = ld2(slab + pp * kStride + 4 * lane);
-      s_g[w][pp][lane] = pack2(red24((d.x >> 16) * gam), red24((d.y >> 16) * gam_neg));
+      for (int pp = 0; pp < P; pp++) {
+        const uint2 d = ld2(slab + pp * kStride + 4 * lane);
+        s_g[w][pp][lane] = pack2(red24((d.x >> 16) * gam), red24((d.y >> 16) * gam_neg));
+      }
+      if constexpr (MODE == kFusedKeygen) {
+        // s_hat = poly_ntt(s_w): the centred representatives.
+#pragma unroll
+        for (int it = 0; it < P / 2; it++) {
+          const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
+          const v4i v = *reinterpret_cast<const v4i *>(slab + pp * kStride + ch * 8);
+          if (pp < nvalid)
+            st_stream(reinterpret_cast<v4i *>(io.out1 + ((i0 + pp) * K + w) * kN) + ch,
+                      barrett_v4(v));
+        }
+      }
     }
     __syncthreads();
 
     // 2. Row w: lane covers instance pp, coefficients 8ch .. 8ch+7.  The
-    // A_hat loads of the next chunk are in flight while this one is summed;
+    // matrix loads of the next chunk are in flight while this one is summed;
     // the scheduling barrier keeps the compiler from hoisting more of them.
-    auto load_row = [&](v4i *a, int it) {
-      const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
-      const v4i *arow =
-          reinterpret_cast<const v4i *>(A + ((i0 + pp) * K + w) * (size_t)(K * kN)) + ch;
-#pragma unroll
-      for (int j = 0; j < K; j++)
-        a[j] = pp < nvalid ? ld_stream(arow + j * (kN / 8)) : v4i{0, 0, 0, 0};
-    };
     uint32_t res[P / 2][4];
-    v4i a_cur[K], a_nxt[K];
-    load_row(a_cur, 0);
+    if (w < R) {
+      auto load_row = [&](v4i *a, int it) {
+        const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
+        const v4i *arow = reinterpret_cast<const v4i *>(row_base(i0 + pp)) + ch;
 #pragma unroll
-    for (int it = 0; it < P / 2; it++) {
-      const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
-      if (it + 1 < P / 2)
-        load_row(a_nxt, it + 1);
-      int r0[4] = {0, 0, 0, 0}, r1[4] = {0, 0, 0, 0};
+        for (int j = 0; j < K; j++)
+          a[j] = pp < nvalid ? ld_stream(arow + j * (kN / 8)) : v4i{0, 0, 0, 0};
+      };
+      v4i a_cur[K], a_nxt[K];
+      load_row(a_cur, 0);
 #pragma unroll
-      for (int j = 0; j < K; j++) {
-        const v4i b = *reinterpret_cast<const v4i *>(s_slab[j] + pp * kStride + ch * 8);
-        const uint2 gg = *reinterpret_cast<const uint2 *>(&s_g[j][pp][2 * ch]);
+      for (int it = 0; it < P / 2; it++) {
+        const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
+        if (it + 1 < P / 2)
+          load_row(a_nxt, it + 1);
+        int r0[4] = {0, 0, 0, 0}, r1[4] = {0, 0, 0, 0};
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
-          const uint32_t bu = (uint32_t)b[u], gs = u < 2 ? gg.x : gg.y;
-          const uint32_t x = __builtin_amdgcn_perm(gs, bu, (u & 1) ? 0x07060100u : 0x05040100u);
-          const uint32_t y = __builtin_amdgcn_alignbit(bu, bu, 16);
-          r0[u] = dot2((uint32_t)a_cur[j][u], x, r0[u]);
-          r1[u] = dot2((uint32_t)a_cur[j][u], y, r1[u]);
+        for (int j = 0; j < K; j++) {
+          const v4i b = *reinterpret_cast<const v4i *>(s_slab[j] + pp * kStride + ch * 8);
+          const uint2 gg = *reinterpret_cast<const uint2 *>(&s_g[j][pp][2 * ch]);
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            const uint32_t bu = (uint32_t)b[u], gs = u < 2 ? gg.x : gg.y;
+            const uint32_t x = __builtin_amdgcn_perm(gs, bu, (u & 1) ? 0x07060100u : 0x05040100u);
+            const uint32_t y = __builtin_amdgcn_alignbit(bu, bu, 16);
+            r0[u] = dot2((uint32_t)a_cur[j][u], x, r0[u]);
+            r1[u] = dot2((uint32_t)a_cur[j][u], y, r1[u]);
+          }
         }
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+          res[it][u] =
+              pack2(red32((uint32_t)r0[u] + kSumOffset), red32((uint32_t)r1[u] + kSumOffset));
+#pragma unroll
+        for (int j = 0; j < K; j++)
+          a_cur[j] = a_nxt[j];
+        __builtin_amdgcn_sched_barrier(0);
       }
-#pragma unroll
-      for (int u = 0; u < 4; u++)
-        res[it][u] = pack2(red32((uint32_t)r0[u] + kSumOffset), red32((uint32_t)r1[u] + kSumOffset));
-#pragma unroll
-      for (int j = 0; j < K; j++)
-        a_cur[j] = a_nxt[j];
-      __builtin_amdgcn_sched_barrier(0);
     }
     __syncthreads();  // every wave is done reading the slabs
 
-    // 3. t_w = invntt(row w), centred.
+    // 3. Output row w.
+    if (w < R) {
+      if constexpr (MODE == kFusedKeygen) {
+        // t_hat_w = reduce(row + poly_ntt(e_w)), NTT domain.
+        tile_in<P>(slab,
+                   [&](int pp) -> const int16_t * {
+                     return pp < nvalid ? io.add0 + ((i0 + pp) * K + w) * kN : nullptr;
+                   },
+                   lane);
+        forward();
 #pragma unroll
-    for (int it = 0; it < P / 2; it++) {
-      const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
-      *reinterpret_cast<v4i *>(slab + pp * kStride + ch * 8) =
-          v4i{(int)res[it][0], (int)res[it][1], (int)res[it][2], (int)res[it][3]};
+        for (int it = 0; it < P / 2; it++) {
+          const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
+          const v4i e = *reinterpret_cast<const v4i *>(slab + pp * kStride + ch * 8);
+          const v4i sum = add16(v4i{(int)res[it][0], (int)res[it][1], (int)res[it][2],
+                                    (int)res[it][3]},
+                                e);  // < 3q: no wrap
+          if (pp < nvalid)
+            st_stream(reinterpret_cast<v4i *>(io.out0 + ((i0 + pp) * K + w) * kN) + ch,
+                      barrett_v4(sum));
+        }
+      } else {
+#pragma unroll
+        for (int it = 0; it < P / 2; it++) {
+          const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
+          *reinterpret_cast<v4i *>(slab + pp * kStride + ch * 8) =
+              v4i{(int)res[it][0], (int)res[it][1], (int)res[it][2], (int)res[it][3]};
+        }
+        wave_sync();
+        pass_c16<kFormInter>(pslab, g, i2lo, i2hi, s_bi2);
+        wave_sync();
+        pass_m16<kFormCentred>(pslab, g, i1lo, i1hi, ib1);
+        wave_sync();
+        if constexpr (MODE == kFusedMatvec) {
+          tile_out<P>(slab,
+                      [&](int pp) -> int16_t * {
+                        return pp < nvalid ? io.out0 + ((i0 + pp) * K + w) * kN : nullptr;
+                      },
+                      lane);
+        } else {
+#pragma unroll
+          for (int it = 0; it < P / 2; it++) {
+            const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
+            if (pp >= nvalid)
+              continue;
+            const size_t inst = i0 + pp;
+            const v4i x = *reinterpret_cast<const v4i *>(slab + pp * kStride + ch * 8);
+            const v4i zero = {0, 0, 0, 0};
+            if constexpr (MODE == kFusedEncrypt) {
+              if (w < K) {  // u_w = reduce(x + e1_w)
+                const v4i e1 =
+                    ld_stream(reinterpret_cast<const v4i *>(io.add0 + (inst * K + w) * kN) + ch);
+                st_stream(reinterpret_cast<v4i *>(io.out0 + (inst * K + w) * kN) + ch,
+                          centred_sum<1>(x, e1, zero));
+              } else {  // v = reduce(x + e2 + m)
+                const v4i e2 = ld_stream(reinterpret_cast<const v4i *>(io.add1 + inst * kN) + ch);
+                const v4i m = io.add2
+                                  ? ld_stream(reinterpret_cast<const v4i *>(io.add2 + inst * kN) + ch)
+                                  : zero;
+                st_stream(reinterpret_cast<v4i *>(io.out1 + inst * kN) + ch,
+                          centred_sum<1>(x, e2, m));
+              }
+            } else {  // decrypt: w = reduce(v - x)
+              const v4i vin = ld_stream(reinterpret_cast<const v4i *>(io.add0 + inst * kN) + ch);
+              st_stream(reinterpret_cast<v4i *>(io.out0 + inst * kN) + ch,
+                        centred_sum<-1>(x, vin, zero));
+            }
+          }
+        }
+      }
+      wave_sync();
     }
-    wave_sync();
-    pass_c16<kFormInter>(slab + tt * kStride, g, i2lo, i2hi, s_bi2);
-    wave_sync();
-    pass_m16<kFormCentred>(slab + tt * kStride, g, i1lo, i1hi, ib1);
-    wave_sync();
-    tile_out<P>(slab,
-                [&](int pp) -> int16_t * {
-                  return pp < nvalid ? t + ((i0 + pp) * K + w) * kN : nullptr;
-                },
-                lane);
-    wave_sync();
   }
 }
 
 }  // namespace
 
-int launch_matvec(int k, int s_bits, int16_t *t, const int16_t *A, const void *s, size_t ninst,
-                  const DeviceTables *tab, int ncu, hipStream_t stream)
+int launch_fused(int k, int mode, int s_bits, const FusedIo &io, const DeviceTables *tab, int ncu,
+                 hipStream_t stream)
 {
-  const size_t ntiles = (ninst + kTileInst - 1) / kTileInst;
+  const size_t ntiles = (io.ninst + kTileInst - 1) / kTileInst;
   auto go = [&](auto kernel, int waves) {
     const size_t cap = (size_t)ncu * resident_per_cu((const void *)kernel, 64 * waves);
     const dim3 grid((unsigned)(ntiles < cap ? ntiles : cap));
-    hipLaunchKernelGGL(kernel, grid, dim3(64 * waves), 0, stream, t, A, s, ninst, tab);
+    hipLaunchKernelGGL(kernel, grid, dim3(64 * waves), 0, stream, io, tab);
     return 0;
   };
-  auto with_k = [&](auto kbits) {
-    constexpr int B = decltype(kbits)::value;
+  auto with_k = [&](auto mode_c, auto bits_c) {
+    constexpr int M = decltype(mode_c)::value, B = decltype(bits_c)::value;
     switch (k) {
-      case 2: return go(k_matvec<2, B>, 2);
-      case 3: return go(k_matvec<3, B>, 3);
-      case 4: return go(k_matvec<4, B>, 4);
+      case 2: return go(k_fused<2, M, B>, fused_waves<M, 2>());
+      case 3: return go(k_fused<3, M, B>, fused_waves<M, 3>());
+      case 4: return go(k_fused<4, M, B>, fused_waves<M, 4>());
       default: return -1;
     }
   };
-  switch (s_bits) {
-    case 0: return with_k(std::integral_constant<int, 0>{});
-    case 2: return with_k(std::integral_constant<int, 2>{});
-    case 3: return with_k(std::integral_constant<int, 3>{});
-    case 4: return with_k(std::integral_constant<int, 4>{});
-    default: return -1;
+  using std::integral_constant;
+  switch (mode) {
+    case kFusedMatvec:
+      switch (s_bits) {
+        case 0: return with_k(integral_constant<int, kFusedMatvec>{}, integral_constant<int, 0>{});
+        case 2: return with_k(integral_constant<int, kFusedMatvec>{}, integral_constant<int, 2>{});
+        case 3: return with_k(integral_constant<int, kFusedMatvec>{}, integral_constant<int, 3>{});
+        case 4: return with_k(integral_constant<int, kFusedMatvec>{}, integral_constant<int, 4>{});
+        default: return -1;
+      }
+    case kFusedKeygen:
+      return s_bits ? -1 : with_k(integral_constant<int, kFusedKeygen>{}, integral_constant<int, 0>{});
+    case kFusedEncrypt:
+      return s_bits ? -1 : with_k(integral_constant<int, kFusedEncrypt>{}, integral_constant<int, 0>{});
+    case kFusedDecrypt:
+      return s_bits ? -1 : with_k(integral_constant<int, kFusedDecrypt>{}, integral_constant<int, 0>{});
+    default:
+      return -1;
   }
 }
 

@@ -68,9 +68,31 @@ int launch_mfma_packed(int shape, int bits, int form, int16_t *out, const uint8_
 int launch_pack_small(int bits, uint8_t *out, const int16_t *in, size_t npoly, int ncu,
                       hipStream_t stream);
 
-// s_bits = 0: s is int16 polys; 2, 3 or 4: s is packed.
-int launch_matvec(int k, int s_bits, int16_t *t, const int16_t *A, const void *s, size_t ninst,
-                  const DeviceTables *tab, int ncu, hipStream_t stream);
+// Modes of the fused module kernel and the buffers each uses (shapes per
+// instance; k = module rank; all polys int16[256] unless packed):
+//   kFusedMatvec:  out0 = t [k] <- invntt(mat [k][k] . ntt(x [k]))
+//   kFusedKeygen:  out0 = t_hat [k], out1 = s_hat [k] <- mat = A_hat [k][k],
+//                  x = s [k], add0 = e [k]
+//   kFusedEncrypt: out0 = u [k], out1 = v [1] <- mat = At_hat [k][k],
+//                  mat2 = t_hat [k], x = r [k], add0 = e1 [k], add1 = e2 [1],
+//                  add2 = m [1] or null
+//   kFusedDecrypt: out0 = w [1] <- mat = s_hat [k], x = u [k], add0 = v [1]
+enum FusedMode { kFusedMatvec = 0, kFusedKeygen = 1, kFusedEncrypt = 2, kFusedDecrypt = 3 };
+struct FusedIo {
+  int16_t *out0;
+  int16_t *out1;
+  const int16_t *mat;
+  const int16_t *mat2;
+  const void *x;  // int16 polys, or packed (matvec only)
+  const int16_t *add0;
+  const int16_t *add1;
+  const int16_t *add2;
+  size_t ninst;
+};
+
+// s_bits = 0: x is int16 polys; 2, 3 or 4: x is packed (kFusedMatvec only).
+int launch_fused(int k, int mode, int s_bits, const FusedIo &io, const DeviceTables *tab, int ncu,
+                 hipStream_t stream);
 
 int launch_basemul(int form, int16_t *r, const int16_t *a, const int16_t *b, size_t npoly,
                    const DeviceTables *tab, int ncu, hipStream_t stream);

@@ -94,6 +94,23 @@ def packed_digests(ref: Reference) -> dict:
     return d
 
 
+def kpke_digests(ref: Reference, ninst: int = 65536, k: int = 3) -> dict:
+    """K-PKE cores on kpke_workload (seed 9): keygen, encryption with the
+    transposed matrix, decryption of that ciphertext."""
+    w = tk.kpke_workload(ninst, k)
+    t_hat, s_hat = ref.kpke_keygen(w["A_hat"], w["s"], w["e"], k)
+    At = np.ascontiguousarray(w["A_hat"].transpose(0, 2, 1, 3))
+    u, v = ref.kpke_encrypt(At, t_hat, w["r"], w["e1"], w["e2"], w["m"], k)
+    dec = ref.kpke_decrypt(s_hat, u, v, k)
+    bits = ((np.mod(dec.astype(np.int64), Q) * 2 + Q // 2) // Q) & 1
+    d = {"ninst": ninst, "k": k, "A_hat": tk.canon_digest(w["A_hat"]),
+         "t_hat": tk.canon_digest(t_hat), "s_hat": tk.canon_digest(s_hat),
+         "u": tk.canon_digest(u), "v": tk.canon_digest(v), "w": tk.canon_digest(dec),
+         "message_bits_recovered": bool(np.array_equal(bits, w["bits"]))}
+    print(f"kpke: {d}", flush=True)
+    return d
+
+
 def digests(ref: Reference) -> dict:
     d = {}
     for cfg in (1, 2, 3):
@@ -127,6 +144,7 @@ def digests(ref: Reference) -> dict:
                     "s": tk.canon_digest(s), "t": tk.canon_digest(t)}
     print(f"config 5: {d['config5']}", flush=True)
     d["packed"] = packed_digests(ref)
+    d["kpke"] = kpke_digests(ref)
     return d
 
 

@@ -90,6 +90,33 @@ def test_packed_validation(tk):
     assert L.matvec_ntt_fused_packed(None, None, None, 3, 0, 3, None) == tk.OK
 
 
+def test_kpke_validation(tk):
+    L = tk.lib()
+    buf = (ctypes.c_int16 * (N * 256 + 16))()
+    al = (ctypes.addressof(buf) + 15) & ~15
+    P = N * 2  # bytes per poly
+    # k out of range; empty batch; null inputs
+    assert L.kpke_keygen_core(al, al + 10 * P, al + 20 * P, al + 60 * P, al + 70 * P, 5, 1,
+                              None) == tk.EINVAL
+    assert L.kpke_keygen_core(None, None, None, None, None, 3, 0, None) == tk.OK
+    assert L.kpke_keygen_core(al, None, al + 20 * P, al + 60 * P, al + 70 * P, 3, 1,
+                              None) == tk.EINVAL
+    # t_hat overlapping A_hat (not an allowed alias), misaligned output
+    assert L.kpke_keygen_core(al + 20 * P, al + 10 * P, al + 20 * P, al + 60 * P, al + 70 * P, 3,
+                              1, None) == tk.EINVAL
+    assert L.kpke_keygen_core(al + 2, al + 10 * P, al + 20 * P, al + 60 * P, al + 70 * P, 3, 1,
+                              None) == tk.EINVAL
+    # u partially overlapping e1; v aliasing m
+    assert L.kpke_encrypt_core(al + P, al + 10 * P, al + 20 * P, al + 40 * P, al + 50 * P, al,
+                               al + 60 * P, al + 70 * P, 3, 1, None) == tk.EINVAL
+    assert L.kpke_encrypt_core(al, al + 70 * P, al + 20 * P, al + 40 * P, al + 50 * P, al + 80 * P,
+                               al + 60 * P, al + 70 * P, 3, 1, None) == tk.EINVAL
+    assert L.kpke_encrypt_core(None, None, None, None, None, None, None, None, 2, 0, None) == tk.OK
+    # w partially overlapping u
+    assert L.kpke_decrypt_core(al + 21 * P, al, al + 20 * P, al + 60 * P, 3, 1, None) == tk.EINVAL
+    assert L.kpke_decrypt_core(al, al, al, al, 1, 1, None) == tk.EINVAL
+
+
 def test_workload_determinism(tk):
     a = tk.wl_uniform(1 << 17, 42)
     b = tk.wl_uniform(1 << 17, 42)

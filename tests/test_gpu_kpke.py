@@ -1,0 +1,119 @@
+"""GPU parity of the fused K-PKE arithmetic cores (keygen, encrypt, decrypt)
+against the oracle's pqcrystals composition: raw int16 equality at ragged
+instance counts and every rank, the allowed in-place aliases, the message
+round trip, and the full-size digests of the compiled reference."""
+import numpy as np
+import pytest
+
+from conftest import centred
+
+pytestmark = pytest.mark.gpu
+
+N, Q = 256, 3329
+
+
+def _dev(a, device):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.int16)).to(device)
+
+
+def _transpose(A):
+    return np.ascontiguousarray(A.transpose(0, 2, 1, 3))
+
+
+def _np(t, shape):
+    return t.cpu().numpy().reshape(shape)
+
+
+@pytest.mark.parametrize("k", [2, 3, 4])
+@pytest.mark.parametrize("ninst", [1, 17, 300])
+def test_kpke_cores_vs_oracle(tk, gpu, oracle, k, ninst):
+    w = tk.kpke_workload(ninst, k, seed=400 + 7 * k + ninst)
+    want_t, want_s = oracle.kpke_keygen(w["A_hat"], w["s"], w["e"], k)
+    t_hat, s_hat = tk.kpke_keygen(_dev(w["A_hat"], gpu), _dev(w["s"], gpu), _dev(w["e"], gpu), k)
+    assert np.array_equal(_np(s_hat, want_s.shape), want_s), f"s_hat k {k} ninst {ninst}"
+    assert np.array_equal(_np(t_hat, want_t.shape), want_t), f"t_hat k {k} ninst {ninst}"
+
+    At = _transpose(w["A_hat"])
+    want_u, want_v = oracle.kpke_encrypt(At, want_t, w["r"], w["e1"], w["e2"], w["m"], k)
+    u, v = tk.kpke_encrypt(_dev(At, gpu), t_hat, _dev(w["r"], gpu), _dev(w["e1"], gpu),
+                           _dev(w["e2"], gpu), _dev(w["m"], gpu), k)
+    assert np.array_equal(_np(u, want_u.shape), want_u), f"u k {k} ninst {ninst}"
+    assert np.array_equal(_np(v, want_v.shape), want_v), f"v k {k} ninst {ninst}"
+    # without a message poly
+    want_u0, want_v0 = oracle.kpke_encrypt(At, want_t, w["r"], w["e1"], w["e2"], None, k)
+    _, v0 = tk.kpke_encrypt(_dev(At, gpu), t_hat, _dev(w["r"], gpu), _dev(w["e1"], gpu),
+                            _dev(w["e2"], gpu), None, k)
+    assert np.array_equal(_np(v0, want_v0.shape), want_v0)
+
+    want_w = oracle.kpke_decrypt(want_s, want_u, want_v, k)
+    dec = tk.kpke_decrypt(s_hat, u, v, k)
+    assert np.array_equal(_np(dec, want_w.shape), want_w), f"w k {k} ninst {ninst}"
+
+
+def test_kpke_large_addends(tk, gpu, oracle):
+    """e1, e2, m and v far outside ML-KEM's ranges but short of overflowing
+    the reference's int16 adds: the sums stay exact."""
+    k, ninst = 3, 40
+    w = tk.kpke_workload(ninst, k, seed=450)
+    e1 = tk.wl_uniform(ninst * k * N, 451, -29000, 29000).reshape(ninst, k, N)
+    e2 = tk.wl_uniform(ninst * N, 452, -14000, 14000).reshape(ninst, N)
+    m = tk.wl_uniform(ninst * N, 453, -14000, 14000).reshape(ninst, N)
+    v_in = tk.wl_uniform(ninst * N, 454, -29000, 29000).reshape(ninst, N)
+    t_hat, s_hat = oracle.kpke_keygen(w["A_hat"], w["s"], w["e"], k)
+    At = _transpose(w["A_hat"])
+    want_u, want_v = oracle.kpke_encrypt(At, t_hat, w["r"], e1, e2, m, k)
+    u, v = tk.kpke_encrypt(_dev(At, gpu), _dev(t_hat, gpu), _dev(w["r"], gpu), _dev(e1, gpu),
+                           _dev(e2, gpu), _dev(m, gpu), k)
+    assert np.array_equal(_np(u, want_u.shape), want_u)
+    assert np.array_equal(_np(v, want_v.shape), want_v)
+    want_w = oracle.kpke_decrypt(s_hat, want_u, v_in, k)
+    dec = tk.kpke_decrypt(_dev(s_hat, gpu), u, _dev(v_in, gpu), k)
+    assert np.array_equal(_np(dec, want_w.shape), want_w)
+
+
+def test_kpke_in_place_aliases(tk, gpu, oracle):
+    """s_hat over s, t_hat over e, u over e1, v over e2, w over v."""
+    k, ninst = 3, 100
+    w = tk.kpke_workload(ninst, k, seed=460)
+    want_t, want_s = oracle.kpke_keygen(w["A_hat"], w["s"], w["e"], k)
+    ds, de = _dev(w["s"], gpu), _dev(w["e"], gpu)
+    tk.kpke_keygen(_dev(w["A_hat"], gpu), ds, de, k, t_hat=de, s_hat=ds)
+    assert np.array_equal(_np(ds, want_s.shape), want_s)
+    assert np.array_equal(_np(de, want_t.shape), want_t)
+    At = _transpose(w["A_hat"])
+    want_u, want_v = oracle.kpke_encrypt(At, want_t, w["r"], w["e1"], w["e2"], w["m"], k)
+    de1, de2 = _dev(w["e1"], gpu), _dev(w["e2"], gpu)
+    tk.kpke_encrypt(_dev(At, gpu), de, _dev(w["r"], gpu), de1, de2, _dev(w["m"], gpu), k,
+                    u=de1, v=de2)
+    assert np.array_equal(_np(de1, want_u.shape), want_u)
+    assert np.array_equal(_np(de2, want_v.shape), want_v)
+    want_w = oracle.kpke_decrypt(want_s, want_u, want_v, k)
+    tk.kpke_decrypt(ds, de1, de2, k, w=de2)
+    assert np.array_equal(_np(de2, want_w.shape), want_w)
+
+
+@pytest.mark.parametrize("k", [2, 3, 4])
+def test_kpke_message_round_trip(tk, gpu, k):
+    w = tk.kpke_workload(1000, k, seed=470 + k)
+    t_hat, s_hat = tk.kpke_keygen(_dev(w["A_hat"], gpu), _dev(w["s"], gpu), _dev(w["e"], gpu), k)
+    u, v = tk.kpke_encrypt(_dev(_transpose(w["A_hat"]), gpu), t_hat, _dev(w["r"], gpu),
+                           _dev(w["e1"], gpu), _dev(w["e2"], gpu), _dev(w["m"], gpu), k)
+    dec = np.mod(tk.kpke_decrypt(s_hat, u, v, k).cpu().numpy().astype(np.int64), Q)
+    assert np.array_equal(((dec * 2 + Q // 2) // Q) & 1, w["bits"])
+
+
+def test_kpke_full_size_digests(tk, gpu, digests):
+    """65,536 instances of rank 3: every output centred and equal, mod q, to
+    the compiled reference's (so raw-equal)."""
+    d = digests["kpke"]
+    k = d["k"]
+    w = tk.kpke_workload(d["ninst"], k)
+    t_hat, s_hat = tk.kpke_keygen(_dev(w["A_hat"], gpu), _dev(w["s"], gpu), _dev(w["e"], gpu), k)
+    u, v = tk.kpke_encrypt(_dev(_transpose(w["A_hat"]), gpu), t_hat, _dev(w["r"], gpu),
+                           _dev(w["e1"], gpu), _dev(w["e2"], gpu), _dev(w["m"], gpu), k)
+    dec = tk.kpke_decrypt(s_hat, u, v, k)
+    for name, t in (("t_hat", t_hat), ("s_hat", s_hat), ("u", u), ("v", v), ("w", dec)):
+        x = t.cpu().numpy()
+        assert np.array_equal(x, centred(x)), f"{name} not centred"
+        assert tk.canon_digest(x) == d[name], name

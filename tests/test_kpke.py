@@ -93,3 +93,17 @@ def test_kpke_message_round_trip(oracle, tk, k):
     w = canon(oracle.kpke_decrypt(s_hat, u, v, k)).astype(np.int64)
     decoded = ((w * 2 + Q // 2) // Q) & 1
     assert np.array_equal(decoded, bits)
+
+
+def test_kpke_full_size_digests(oracle, tk, digests):
+    """65,536 instances of rank 3 against the compiled reference's digests."""
+    d = digests["kpke"]
+    k = d["k"]
+    w = tk.kpke_workload(d["ninst"], k)
+    assert tk.canon_digest(w["A_hat"]) == d["A_hat"]
+    t_hat, s_hat = oracle.kpke_keygen(w["A_hat"], w["s"], w["e"], k)
+    assert (tk.canon_digest(t_hat), tk.canon_digest(s_hat)) == (d["t_hat"], d["s_hat"])
+    u, v = oracle.kpke_encrypt(_transpose(w["A_hat"]), t_hat, w["r"], w["e1"], w["e2"], w["m"], k)
+    assert (tk.canon_digest(u), tk.canon_digest(v)) == (d["u"], d["v"])
+    assert tk.canon_digest(oracle.kpke_decrypt(s_hat, u, v, k)) == d["w"]
+    assert d["message_bits_recovered"]
