@@ -340,6 +340,42 @@ def bench_kpke(tk, torch, steps, ninst=65536, k=3):
     return out
 
 
+def bench_host(tk, torch, npoly=1 << 20, reps=3):
+    """Host-resident batches (PCIe-inclusive): 2^20 polys of the edge-value
+    mix streamed through tk_host_pipeline (forward NTT, out of place), pinned
+    host buffers over a few chunk/depth shapes, and pageable buffers at the
+    default shape.  Wall-clock per run, best of `reps`."""
+    x = tk.workload(3, npoly)
+    hin = torch.from_numpy(x).pin_memory()
+    hout = torch.empty_like(hin).pin_memory()
+    res = {"npoly": npoly, "bytes_each_way": npoly * 512}
+
+    def run(p, a, b):
+        best = 1e30
+        p.run(a, b, direction=0)  # warm
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            p.run(a, b, direction=0)
+            best = min(best, time.perf_counter() - t0)
+        return best
+
+    shapes = {}
+    for chunk, depth in ((8192, 3), (16384, 2), (16384, 3), (16384, 4), (65536, 3)):
+        p = tk.HostPipeline(chunk, depth)
+        t = run(p, hin, hout)
+        p.close()
+        shapes[f"chunk{chunk}_depth{depth}"] = {"ms": t * 1e3, "poly_s": npoly / t,
+                                                "GBs_each_way": npoly * 512 / t / 1e9}
+    res["pinned"] = shapes
+    p = tk.HostPipeline()
+    xo = np.empty_like(x)
+    t = run(p, x, xo)
+    p.close()
+    res["pageable_default"] = {"ms": t * 1e3, "poly_s": npoly / t,
+                               "GBs_each_way": npoly * 512 / t / 1e9}
+    return res
+
+
 def check_parity(tk, torch, method, x, buf_fwd, buf_rt):
     """Canonical equality of the GPU forward and inverse against the oracle."""
     from oracle.oracle import Oracle
@@ -430,6 +466,8 @@ def main():
         extras["packed_input"] = bench_packed(tk, torch, args.steps)
         print("[bench] K-PKE cores", file=sys.stderr, flush=True)
         extras["kpke_cores"] = bench_kpke(tk, torch, args.steps)
+        print("[bench] host-resident batches (PCIe)", file=sys.stderr, flush=True)
+        extras["host_pipeline"] = bench_host(tk, torch)
 
     cpu = None
     if not args.no_cpu:

@@ -139,6 +139,34 @@ int tk_ragged_plan_destroy(tk_ragged_plan *plan);
 int matvec_ntt_fused(int16_t *t, const int16_t *A_hat, const int16_t *s, int k, size_t ninst,
                      void *stream);
 
+/* ---- host-resident batches (PCIe streaming) ---------------------------- */
+
+/* The reference calls ntt()/invntt() per poly on host memory (ref/ntt.h:14-15,
+ * ntt_bench.cu:947-956).  A host pipeline streams a host batch through the
+ * device in chunks of chunk_polys: each chunk is copied in, transformed in
+ * place and copied out on its own stream, so up to depth chunks are in
+ * flight and the copies of both directions overlap the kernels.  Host
+ * buffers from hipHostMalloc (or torch pin_memory) are moved by DMA
+ * directly; pageable buffers go through the HIP runtime's staging (slower).
+ * Creation allocates depth device buffers and streams on the current
+ * device; run blocks until out is written, on the creating device only. */
+#define TK_HOST_MAX_DEPTH 4
+#define TK_HOST_DEFAULT_CHUNK 16384 /* polys (8 MiB) per chunk */
+typedef struct tk_host_pipeline tk_host_pipeline;
+int tk_host_pipeline_create(tk_host_pipeline **pipeline, size_t chunk_polys, int depth);
+/* direction 0: forward (default form canonical), 1: inverse (centred);
+ * method and form as for ntt_batch_ex; out may equal in (no partial
+ * overlap).  One run at a time per pipeline (others wait). */
+int tk_host_pipeline_run(tk_host_pipeline *pipeline, int16_t *out, const int16_t *in,
+                         size_t npoly, int direction, int method, int form);
+int tk_host_pipeline_destroy(tk_host_pipeline *pipeline);
+
+/* The reference's loop `for (i..n) ntt(poly[i])` on a host array, in place,
+ * through a per-device default pipeline (chunk TK_HOST_DEFAULT_CHUNK, depth
+ * 3), with the default forms (canonical forward, centred inverse). */
+int ntt_batch_host(int16_t *polys, size_t npoly);
+int invntt_batch_host(int16_t *polys, size_t npoly);
+
 /* ---- K-PKE arithmetic cores -------------------------------------------- */
 
 /* The arithmetic of ML-KEM's K-PKE around the NTT path, one fused kernel

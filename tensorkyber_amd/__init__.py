@@ -93,6 +93,11 @@ def _declare(lib):
         "pack_small_batch": ([vp, vp, sz, i, vp], i),
         "ntt_batch_packed": ([vp, vp, sz, i, i, i, vp], i),
         "matvec_ntt_fused_packed": ([vp, vp, vp, i, sz, i, vp], i),
+        "tk_host_pipeline_create": ([ctypes.POINTER(vp), sz, i], i),
+        "tk_host_pipeline_run": ([vp, vp, vp, sz, i, i, i], i),
+        "tk_host_pipeline_destroy": ([vp], i),
+        "ntt_batch_host": ([vp, sz], i),
+        "invntt_batch_host": ([vp, sz], i),
         "kpke_keygen_core": ([vp, vp, vp, vp, vp, i, sz, vp], i),
         "kpke_encrypt_core": ([vp, vp, vp, vp, vp, vp, vp, vp, i, sz, vp], i),
         "kpke_decrypt_core": ([vp, vp, vp, vp, i, sz, vp], i),
@@ -277,6 +282,50 @@ def matvec_packed(A_hat, s_packed, k: int, bits: int, out=None, stream=None):
     return out
 
 
+class HostPipeline:
+    """Streams host-resident poly batches through the device in chunks
+    (tk_host_pipeline_*).  Buffers: numpy int16 arrays or CPU torch tensors
+    (pinned ones, torch.empty(..., pin_memory=True), take the DMA path)."""
+
+    def __init__(self, chunk_polys: int = 16384, depth: int = 3):
+        self._p = ctypes.c_void_p()
+        _check(lib().tk_host_pipeline_create(ctypes.byref(self._p), chunk_polys, depth),
+               "tk_host_pipeline_create")
+
+    @staticmethod
+    def _host(a, name):
+        torch = _torch()
+        if isinstance(a, torch.Tensor):
+            if a.is_cuda or a.dtype != torch.int16 or not a.is_contiguous() or a.numel() % N:
+                raise ValueError(f"{name} must be a contiguous CPU int16 tensor of whole polys")
+            return a.data_ptr(), a.numel() // N
+        if not isinstance(a, np.ndarray) or a.dtype != np.int16 or not a.flags["C_CONTIGUOUS"] \
+                or a.size % N:
+            raise ValueError(f"{name} must be a contiguous int16 array of whole polys")
+        return a.ctypes.data, a.size // N
+
+    def run(self, inp, out=None, direction: int = 0, method="auto", form="default"):
+        out = inp if out is None else out
+        pi, n = self._host(inp, "in")
+        po, n_out = self._host(out, "out")
+        if n_out != n:
+            raise ValueError("in and out differ in size")
+        _check(lib().tk_host_pipeline_run(self._p, po, pi, n, direction, _method(method),
+                                          _form(form)), "tk_host_pipeline_run")
+        return out
+
+    def close(self):
+        if self._p:
+            lib().tk_host_pipeline_destroy(self._p)
+            self._p = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def _ptr_or_none(t):
     return None if t is None else t.data_ptr()
 
@@ -452,7 +501,7 @@ def packed_workload(bits: int, npoly: int = 1 << 20) -> np.ndarray:
 __all__ = [
     "N", "Q", "TkError", "load_library", "lib", "ntt", "invntt", "basemul", "matvec",
     "packed_poly_bytes", "pack_small", "ntt_packed", "matvec_packed",
-    "kpke_keygen", "kpke_encrypt", "kpke_decrypt",
+    "kpke_keygen", "kpke_encrypt", "kpke_decrypt", "HostPipeline",
     "host_tables", "host_mfma_operands", "workload", "matvec_workload", "packed_workload",
     "kpke_workload",
     "wl_uniform", "wl_edge_mix", "wl_cbd2", "wl_ragged_layout", "canon_digest", "TkRequest",

@@ -27,6 +27,10 @@ bool aligned16(const void *p);
 bool partial_overlap(const void *a, const void *b, size_t bytes);
 int resolve_method(int method, int form, bool inverse, int *out_method, int *out_form);
 int device_form(int form);
+// Validated batch transform on device buffers (the body of ntt_batch_ex /
+// invntt_batch_ex).
+int transform(bool inverse, int16_t *out, const int16_t *in, size_t npoly, int method, int form,
+              void *stream);
 const HostMatrices &host_matrices();
 const DeviceTables &host_device_tables();
 

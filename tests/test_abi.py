@@ -117,6 +117,20 @@ def test_kpke_validation(tk):
     assert L.kpke_decrypt_core(al, al, al, al, 1, 1, None) == tk.EINVAL
 
 
+def test_host_pipeline_validation(tk):
+    L = tk.lib()
+    p = ctypes.c_void_p()
+    assert L.tk_host_pipeline_create(ctypes.byref(p), 0, 3) == tk.EINVAL
+    assert L.tk_host_pipeline_create(ctypes.byref(p), 1024, 0) == tk.EINVAL
+    assert L.tk_host_pipeline_create(ctypes.byref(p), 1024, 5) == tk.EINVAL
+    assert L.tk_host_pipeline_create(None, 1024, 2) == tk.EINVAL
+    assert L.tk_host_pipeline_run(None, None, None, 4, 0, 0, 0) == tk.EINVAL
+    assert L.tk_host_pipeline_destroy(None) == tk.EINVAL
+    assert L.ntt_batch_host(None, 5) == tk.EINVAL
+    assert L.invntt_batch_host(None, 5) == tk.EINVAL
+    assert L.ntt_batch_host(None, 0) == tk.OK
+
+
 def test_workload_determinism(tk):
     a = tk.wl_uniform(1 << 17, 42)
     b = tk.wl_uniform(1 << 17, 42)
