@@ -376,6 +376,79 @@ def bench_host(tk, torch, npoly=1 << 20, reps=3):
     return res
 
 
+def bench_sched(tk, torch, steps, nreq=3000, seed=11):
+    """Serving mix for the rank-aware fused plan: nreq encryption requests of
+    1..8 instances at ranks 2, 3, 4 (ML-KEM-512/768/1024) drawn with seed 11,
+    each with its own buffers (slices of per-rank pools).  One plan (graph
+    replay) against one kpke_encrypt_core call per request."""
+    import ctypes
+    ranks = tk.wl_uniform(nreq, seed, 2, 5)
+    sizes = tk.wl_uniform(nreq, seed + 1, 1, 9)
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to("cuda:0")  # noqa: E731
+    pools = {}
+    for k in (2, 3, 4):
+        n = int(sizes[ranks == k].sum())
+        w = tk.kpke_workload(n, k, seed=seed + k)
+        t_hat, _ = tk.kpke_keygen(dev(w["A_hat"]), dev(w["s"]), dev(w["e"]), k)
+        pools[k] = {"mat": dev(w["A_hat"].transpose(0, 2, 1, 3)), "mat2": t_hat,
+                    "x": dev(w["r"]), "add0": dev(w["e1"]), "add1": dev(w["e2"]),
+                    "add2": dev(w["m"]), "out0": torch.empty_like(dev(w["r"])),
+                    "out1": torch.empty_like(dev(w["e2"])), "next": 0}
+    polys = {"mat": lambda k: k * k, "mat2": lambda k: k, "x": lambda k: k, "add0": lambda k: k,
+             "add1": lambda k: 1, "add2": lambda k: 1, "out0": lambda k: k, "out1": lambda k: 1}
+    table = (tk.TkFusedRequest * nreq)()
+    calls = []
+    for r in range(nreq):
+        k, n = int(ranks[r]), int(sizes[r])
+        pool = pools[k]
+        off = pool["next"]
+        pool["next"] += n
+        t = table[r]
+        t.mode, t.k, t.ninst = tk.FUSED_ENCRYPT, k, n
+        ptrs = {}
+        for name, per in polys.items():
+            ptrs[name] = pool[name].data_ptr() + off * per(k) * 512
+            setattr(t, name, ptrs[name])
+        calls.append((k, n, ptrs))
+    plan = ctypes.c_void_p()
+    rc = tk.lib().tk_fused_plan_create(ctypes.byref(plan), ctypes.addressof(table), nreq)
+    if rc != 0:
+        raise SystemExit(f"tk_fused_plan_create failed: {rc}")
+    stream = torch.cuda.current_stream()
+    sp = ctypes.c_void_p(stream.cuda_stream)
+    L = tk.lib()
+
+    def singles():
+        for k, n, p in calls:
+            L.kpke_encrypt_core(p["out0"], p["out1"], p["mat"], p["mat2"], p["x"], p["add0"],
+                                p["add1"], p["add2"], k, n, sp)
+
+    def timed(fn, reps):
+        warm(torch, fn)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps, (time.perf_counter() - t0) * 1e3 / reps
+
+    ninst = int(sizes.sum())
+    g_ms, g_wall = timed(lambda: L.tk_fused_plan_execute(plan, sp), steps)
+    d_ms, d_wall = timed(lambda: L.tk_fused_plan_launch(plan, sp), steps)
+    s_ms, s_wall = timed(singles, max(steps // 10, 2))
+    L.tk_fused_plan_destroy(plan)
+    out = {"workload": f"{nreq} encryption requests of 1..8 instances, ranks 2/3/4, seed {seed}",
+           "requests": nreq, "instances": ninst,
+           "plan_graph": {"ms": g_ms, "wall_ms": g_wall, "inst_s": ninst / (g_ms / 1e3)},
+           "plan_direct": {"ms": d_ms, "wall_ms": d_wall},
+           "one_call_per_request": {"ms": s_ms, "wall_ms": s_wall,
+                                    "inst_s": ninst / (max(s_ms, s_wall) / 1e3)}}
+    out["speedup_vs_one_call_per_request"] = max(s_ms, s_wall) / max(g_ms, g_wall)
+    return out
+
+
 def check_parity(tk, torch, method, x, buf_fwd, buf_rt):
     """Canonical equality of the GPU forward and inverse against the oracle."""
     from oracle.oracle import Oracle
@@ -466,6 +539,8 @@ def main():
         extras["packed_input"] = bench_packed(tk, torch, args.steps)
         print("[bench] K-PKE cores", file=sys.stderr, flush=True)
         extras["kpke_cores"] = bench_kpke(tk, torch, args.steps)
+        print("[bench] rank-aware fused plan", file=sys.stderr, flush=True)
+        extras["fused_plan_mix"] = bench_sched(tk, torch, args.steps)
         print("[bench] host-resident batches (PCIe)", file=sys.stderr, flush=True)
         extras["host_pipeline"] = bench_host(tk, torch)
 

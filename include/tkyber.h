@@ -205,6 +205,52 @@ int kpke_encrypt_core(int16_t *u, int16_t *v, const int16_t *At_hat, const int16
 int kpke_decrypt_core(int16_t *w, const int16_t *s_hat, const int16_t *u, const int16_t *v, int k,
                       size_t ninst, void *stream);
 
+/* ---- rank-aware scheduling of mixed fused requests --------------------- */
+
+/* A serving batch mixes ML-KEM-512/768/1024 (module ranks k = 2/3/4,
+ * ref/params.h:10-18) and operations.  A request is a contiguous batch of
+ * ninst instances of one operation at one rank, with the buffers of the
+ * matching single call (which the request's roles map onto):
+ *   TK_FUSED_MATVEC  (matvec_ntt_fused):  out0 = t, mat = A_hat, x = s
+ *   TK_FUSED_KEYGEN  (kpke_keygen_core):  out0 = t_hat, out1 = s_hat,
+ *                                         mat = A_hat, x = s, add0 = e
+ *   TK_FUSED_ENCRYPT (kpke_encrypt_core): out0 = u, out1 = v, mat = At_hat,
+ *                                         mat2 = t_hat, x = r, add0 = e1,
+ *                                         add1 = e2, add2 = m (or NULL)
+ *   TK_FUSED_DECRYPT (kpke_decrypt_core): out0 = w, mat = s_hat, x = u,
+ *                                         add0 = v
+ * Unused roles are NULL.  Each request obeys the rules of its single call;
+ * different requests' outputs must not overlap any other request's buffers.
+ * The plan groups the instances by (operation, rank) -- one kernel per group
+ * through a device table of per-instance pointers -- and records the groups,
+ * largest first, as parallel branches of one hipGraph.  Creation validates,
+ * allocates and synchronises; execute replays the graph (NULL stream = the
+ * legacy default stream), launch issues the group kernels in order. */
+#define TK_FUSED_MATVEC 0
+#define TK_FUSED_KEYGEN 1
+#define TK_FUSED_ENCRYPT 2
+#define TK_FUSED_DECRYPT 3
+typedef struct {
+  int mode;
+  int k;
+  size_t ninst;
+  int16_t *out0;
+  int16_t *out1;
+  const int16_t *mat;
+  const int16_t *mat2;
+  const int16_t *x;
+  const int16_t *add0;
+  const int16_t *add1;
+  const int16_t *add2;
+} tk_fused_request;
+typedef struct tk_fused_plan tk_fused_plan;
+int tk_fused_plan_create(tk_fused_plan **plan, const tk_fused_request *reqs, size_t nreq);
+int tk_fused_plan_execute(tk_fused_plan *plan, void *stream);
+int tk_fused_plan_launch(tk_fused_plan *plan, void *stream);
+/* Number of (operation, rank) groups and of instances in the plan. */
+int tk_fused_plan_info(const tk_fused_plan *plan, int *ngroups, size_t *ninst_total);
+int tk_fused_plan_destroy(tk_fused_plan *plan);
+
 /* ---- packed small-coefficient input ------------------------------------ */
 
 /* The packed form of the reference's TC-PACK path (ntt_bench.cu:278-391,

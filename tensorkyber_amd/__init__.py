@@ -62,6 +62,16 @@ class TkRequest(ctypes.Structure):
                 ("k", ctypes.c_uint16), ("kind", ctypes.c_uint16)]
 
 
+class TkFusedRequest(ctypes.Structure):
+    _fields_ = [("mode", ctypes.c_int), ("k", ctypes.c_int), ("ninst", ctypes.c_size_t),
+                ("out0", ctypes.c_void_p), ("out1", ctypes.c_void_p), ("mat", ctypes.c_void_p),
+                ("mat2", ctypes.c_void_p), ("x", ctypes.c_void_p), ("add0", ctypes.c_void_p),
+                ("add1", ctypes.c_void_p), ("add2", ctypes.c_void_p)]
+
+
+FUSED_MATVEC, FUSED_KEYGEN, FUSED_ENCRYPT, FUSED_DECRYPT = 0, 1, 2, 3
+
+
 class TkWlRequest(ctypes.Structure):
     _fields_ = [("offset", ctypes.c_uint64), ("npoly", ctypes.c_uint32),
                 ("k", ctypes.c_uint16), ("kind", ctypes.c_uint16)]
@@ -93,6 +103,11 @@ def _declare(lib):
         "pack_small_batch": ([vp, vp, sz, i, vp], i),
         "ntt_batch_packed": ([vp, vp, sz, i, i, i, vp], i),
         "matvec_ntt_fused_packed": ([vp, vp, vp, i, sz, i, vp], i),
+        "tk_fused_plan_create": ([ctypes.POINTER(vp), vp, sz], i),
+        "tk_fused_plan_execute": ([vp, vp], i),
+        "tk_fused_plan_launch": ([vp, vp], i),
+        "tk_fused_plan_info": ([vp, ctypes.POINTER(i), ctypes.POINTER(sz)], i),
+        "tk_fused_plan_destroy": ([vp], i),
         "tk_host_pipeline_create": ([ctypes.POINTER(vp), sz, i], i),
         "tk_host_pipeline_run": ([vp, vp, vp, sz, i, i, i], i),
         "tk_host_pipeline_destroy": ([vp], i),
@@ -501,7 +516,7 @@ def packed_workload(bits: int, npoly: int = 1 << 20) -> np.ndarray:
 __all__ = [
     "N", "Q", "TkError", "load_library", "lib", "ntt", "invntt", "basemul", "matvec",
     "packed_poly_bytes", "pack_small", "ntt_packed", "matvec_packed",
-    "kpke_keygen", "kpke_encrypt", "kpke_decrypt", "HostPipeline",
+    "kpke_keygen", "kpke_encrypt", "kpke_decrypt", "HostPipeline", "TkFusedRequest",
     "host_tables", "host_mfma_operands", "workload", "matvec_workload", "packed_workload",
     "kpke_workload",
     "wl_uniform", "wl_edge_mix", "wl_cbd2", "wl_ragged_layout", "canon_digest", "TkRequest",

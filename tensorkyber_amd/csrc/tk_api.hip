@@ -186,6 +186,103 @@ int transform(bool inverse, int16_t *out, const int16_t *in, size_t npoly, int m
   return after_launch();
 }
 
+namespace {
+
+// One buffer of a fused call: address, size and, for an output, the one
+// input it may alias exactly (the kernel reads that input's row in the same
+// lane before it writes the output row at the same address).
+struct Buf {
+  const void *p;
+  size_t bytes;
+  const void *may_alias;
+};
+
+bool bufs_ok(const Buf *out, int nout, const Buf *in, int nin)
+{
+  auto overlap = [](const Buf &a, const Buf &b) {
+    const uintptr_t x = (uintptr_t)a.p, y = (uintptr_t)b.p;
+    return a.p && b.p && x < y + b.bytes && y < x + a.bytes;
+  };
+  for (int i = 0; i < nout; i++) {
+    if (!aligned16(out[i].p))
+      return false;
+    for (int j = i + 1; j < nout; j++)
+      if (overlap(out[i], out[j]))
+        return false;
+    for (int j = 0; j < nin; j++)
+      if (overlap(out[i], in[j]) && !(out[i].may_alias == in[j].p && out[i].p == in[j].p))
+        return false;
+  }
+  for (int j = 0; j < nin; j++)
+    if (in[j].p && !aligned16(in[j].p))
+      return false;
+  return true;
+}
+
+}  // namespace
+
+int validate_fused(int k, int mode, const FusedIo &io)
+{
+  if (k < 2 || k > 4 || io.inst)
+    return TK_EINVAL;
+  if (io.ninst > kMaxPolys / 16)
+    return TK_EINVAL;
+  const size_t pb = io.ninst * kN * sizeof(int16_t), vb = pb * k, mb = vb * k;
+  const int16_t *x = static_cast<const int16_t *>(io.x);
+  switch (mode) {
+    case kFusedMatvec: {
+      if (!io.out0 || !io.mat || !x)
+        return TK_EINVAL;
+      const Buf out[1] = {{io.out0, vb, x}};
+      const Buf in[2] = {{io.mat, mb, nullptr}, {x, vb, nullptr}};
+      return bufs_ok(out, 1, in, 2) ? TK_OK : TK_EINVAL;
+    }
+    case kFusedKeygen: {
+      if (!io.out0 || !io.out1 || !io.mat || !x || !io.add0)
+        return TK_EINVAL;
+      const Buf out[2] = {{io.out0, vb, io.add0}, {io.out1, vb, x}};
+      const Buf in[3] = {{io.mat, mb, nullptr}, {x, vb, nullptr}, {io.add0, vb, nullptr}};
+      return bufs_ok(out, 2, in, 3) ? TK_OK : TK_EINVAL;
+    }
+    case kFusedEncrypt: {
+      if (!io.out0 || !io.out1 || !io.mat || !io.mat2 || !x || !io.add0 || !io.add1)
+        return TK_EINVAL;
+      if (io.add2 && io.add2 == io.out1)  // v may alias e2 only
+        return TK_EINVAL;
+      const Buf out[2] = {{io.out0, vb, io.add0}, {io.out1, pb, io.add1}};
+      const Buf in[6] = {{io.mat, mb, nullptr}, {io.mat2, vb, nullptr}, {x, vb, nullptr},
+                         {io.add0, vb, nullptr}, {io.add1, pb, nullptr}, {io.add2, pb, nullptr}};
+      return bufs_ok(out, 2, in, 6) ? TK_OK : TK_EINVAL;
+    }
+    case kFusedDecrypt: {
+      if (!io.out0 || !io.mat || !x || !io.add0)
+        return TK_EINVAL;
+      const Buf out[1] = {{io.out0, pb, io.add0}};
+      const Buf in[3] = {{io.mat, vb, nullptr}, {x, vb, nullptr}, {io.add0, pb, nullptr}};
+      return bufs_ok(out, 1, in, 3) ? TK_OK : TK_EINVAL;
+    }
+    default:
+      return TK_EINVAL;
+  }
+}
+
+int run_fused_checked(int k, int mode, const FusedIo &io, void *stream)
+{
+  if (k < 2 || k > 4)
+    return TK_EINVAL;
+  if (io.ninst == 0)
+    return TK_OK;
+  int rc = validate_fused(k, mode, io);
+  if (rc)
+    return rc;
+  DevState *st = nullptr;
+  if ((rc = init_device(&st)))
+    return rc;
+  if (launch_fused(k, mode, 0, io, st->tables, st->ncu, (hipStream_t)stream))
+    return TK_ENOTSUP;
+  return after_launch();
+}
+
 }  // namespace tk
 
 using namespace tk;
@@ -269,92 +366,18 @@ int basemul_batch(int16_t *r, const int16_t *a, const int16_t *b, size_t npoly, 
 int matvec_ntt_fused(int16_t *t, const int16_t *A_hat, const int16_t *s, int k, size_t ninst,
                      void *stream)
 {
-  if (k < 2 || k > 4)
-    return TK_EINVAL;
-  if (ninst == 0)
-    return TK_OK;
-  if (!aligned16(t) || !aligned16(A_hat) || !aligned16(s) || ninst > kMaxPolys / 16)
-    return TK_EINVAL;
-  const size_t vb = ninst * k * kN * sizeof(int16_t), mb = vb * k;
-  // t may alias s exactly (each tile reads all of its s before writing t) but
-  // must not overlap A_hat or straddle s.
-  const uintptr_t tp = (uintptr_t)t, ap = (uintptr_t)A_hat, sp = (uintptr_t)s;
-  if ((tp < ap + mb && ap < tp + vb) || (tp != sp && tp < sp + vb && sp < tp + vb))
-    return TK_EINVAL;
-  DevState *st = nullptr;
-  int rc = init_device(&st);
-  if (rc)
-    return rc;
+  // t may alias s exactly: each tile reads all of its s before writing t.
   FusedIo io{};
   io.out0 = t;
   io.mat = A_hat;
   io.x = s;
   io.ninst = ninst;
-  if (launch_fused(k, kFusedMatvec, 0, io, st->tables, st->ncu, (hipStream_t)stream))
-    return TK_ENOTSUP;
-  return after_launch();
+  return run_fused_checked(k, kFusedMatvec, io, stream);
 }
-
-namespace {
-
-// One buffer of a fused K-PKE call: its address, size and, for an output,
-// the one input it may alias exactly (the kernel reads that input's row in
-// the same lane before it writes the output row at the same address).
-struct Buf {
-  const void *p;
-  size_t bytes;
-  const void *may_alias;
-};
-
-bool bufs_ok(const Buf *out, int nout, const Buf *in, int nin)
-{
-  auto overlap = [](const Buf &a, const Buf &b) {
-    const uintptr_t x = (uintptr_t)a.p, y = (uintptr_t)b.p;
-    return a.p && b.p && x < y + b.bytes && y < x + a.bytes;
-  };
-  for (int i = 0; i < nout; i++) {
-    if (!aligned16(out[i].p))
-      return false;
-    for (int j = i + 1; j < nout; j++)
-      if (overlap(out[i], out[j]))
-        return false;
-    for (int j = 0; j < nin; j++)
-      if (overlap(out[i], in[j]) && !(out[i].may_alias == in[j].p && out[i].p == in[j].p))
-        return false;
-  }
-  for (int j = 0; j < nin; j++)
-    if (in[j].p && !aligned16(in[j].p))
-      return false;
-  return true;
-}
-
-int run_fused(int k, int mode, const FusedIo &io, void *stream)
-{
-  DevState *st = nullptr;
-  int rc = init_device(&st);
-  if (rc)
-    return rc;
-  if (launch_fused(k, mode, 0, io, st->tables, st->ncu, (hipStream_t)stream))
-    return TK_ENOTSUP;
-  return after_launch();
-}
-
-}  // namespace
 
 int kpke_keygen_core(int16_t *t_hat, int16_t *s_hat, const int16_t *A_hat, const int16_t *s,
                      const int16_t *e, int k, size_t ninst, void *stream)
 {
-  if (k < 2 || k > 4)
-    return TK_EINVAL;
-  if (ninst == 0)
-    return TK_OK;
-  if (!t_hat || !s_hat || !A_hat || !s || !e || ninst > kMaxPolys / 16)
-    return TK_EINVAL;
-  const size_t vb = ninst * k * kN * sizeof(int16_t);
-  const Buf out[2] = {{t_hat, vb, e}, {s_hat, vb, s}};
-  const Buf in[3] = {{A_hat, vb * k, nullptr}, {s, vb, nullptr}, {e, vb, nullptr}};
-  if (!bufs_ok(out, 2, in, 3))
-    return TK_EINVAL;
   FusedIo io{};
   io.out0 = t_hat;
   io.out1 = s_hat;
@@ -362,27 +385,13 @@ int kpke_keygen_core(int16_t *t_hat, int16_t *s_hat, const int16_t *A_hat, const
   io.x = s;
   io.add0 = e;
   io.ninst = ninst;
-  return run_fused(k, kFusedKeygen, io, stream);
+  return run_fused_checked(k, kFusedKeygen, io, stream);
 }
 
 int kpke_encrypt_core(int16_t *u, int16_t *v, const int16_t *At_hat, const int16_t *t_hat,
                       const int16_t *r, const int16_t *e1, const int16_t *e2, const int16_t *m,
                       int k, size_t ninst, void *stream)
 {
-  if (k < 2 || k > 4)
-    return TK_EINVAL;
-  if (ninst == 0)
-    return TK_OK;
-  if (!u || !v || !At_hat || !t_hat || !r || !e1 || !e2 || ninst > kMaxPolys / 16)
-    return TK_EINVAL;
-  const size_t pb = ninst * kN * sizeof(int16_t), vb = pb * k;
-  const Buf out[2] = {{u, vb, e1}, {v, pb, e2}};
-  const Buf in[6] = {{At_hat, vb * k, nullptr}, {t_hat, vb, nullptr}, {r, vb, nullptr},
-                     {e1, vb, nullptr},         {e2, pb, nullptr},    {m, pb, nullptr}};
-  if (!bufs_ok(out, 2, in, 6))
-    return TK_EINVAL;
-  if (m && m == v)  // v may alias e2 only: m is read after e2's row is added
-    return TK_EINVAL;
   FusedIo io{};
   io.out0 = u;
   io.out1 = v;
@@ -393,30 +402,19 @@ int kpke_encrypt_core(int16_t *u, int16_t *v, const int16_t *At_hat, const int16
   io.add1 = e2;
   io.add2 = m;
   io.ninst = ninst;
-  return run_fused(k, kFusedEncrypt, io, stream);
+  return run_fused_checked(k, kFusedEncrypt, io, stream);
 }
 
 int kpke_decrypt_core(int16_t *w, const int16_t *s_hat, const int16_t *u, const int16_t *v, int k,
                       size_t ninst, void *stream)
 {
-  if (k < 2 || k > 4)
-    return TK_EINVAL;
-  if (ninst == 0)
-    return TK_OK;
-  if (!w || !s_hat || !u || !v || ninst > kMaxPolys / 16)
-    return TK_EINVAL;
-  const size_t pb = ninst * kN * sizeof(int16_t), vb = pb * k;
-  const Buf out[1] = {{w, pb, v}};
-  const Buf in[3] = {{s_hat, vb, nullptr}, {u, vb, nullptr}, {v, pb, nullptr}};
-  if (!bufs_ok(out, 1, in, 3))
-    return TK_EINVAL;
   FusedIo io{};
   io.out0 = w;
   io.mat = s_hat;
   io.x = u;
   io.add0 = v;
   io.ninst = ninst;
-  return run_fused(k, kFusedDecrypt, io, stream);
+  return run_fused_checked(k, kFusedDecrypt, io, stream);
 }
 
 size_t tk_packed_poly_bytes(int bits)

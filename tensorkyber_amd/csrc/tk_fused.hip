@@ -141,15 +141,37 @@ constexpr int fused_rows()
   return MODE == kFusedEncrypt ? K + 1 : (MODE == kFusedDecrypt ? 1 : K);
 }
 
+// Buffer roles of FusedIo / FusedInst, in FusedInst's order.
+enum Role { rOut0 = 0, rOut1, rMat, rMat2, rX, rAdd0, rAdd1, rAdd2 };
+
+// Per-instance stride of each role in the contiguous layout, in int16.
+template <int MODE, int K>
+__device__ __forceinline__ size_t role_stride(int role)
+{
+  switch (role) {
+    case rOut0: return MODE == kFusedDecrypt ? kN : K * kN;
+    case rOut1: return MODE == kFusedEncrypt ? kN : K * kN;
+    case rMat: return MODE == kFusedDecrypt ? K * kN : K * K * kN;
+    case rMat2:
+    case rX: return K * kN;
+    case rAdd0: return MODE == kFusedDecrypt ? kN : K * kN;
+    default: return kN;
+  }
+}
+
 // K: module rank; MODE: kFusedMatvec / Keygen / Encrypt / Decrypt (FusedIo
 // documents the buffers of each); SBITS = 0: the transformed vector holds
 // int16 polys [ninst][K][256]; 2, 3, 4: the packed small-coefficient form
-// [ninst][K][packed bytes].
-template <int K, int MODE, int SBITS>
+// [ninst][K][packed bytes].  TABLE: each instance's buffers come from the
+// per-instance table io.inst (int16 x only); the tile's 16 rows of pointers
+// are parked in LDS, double-buffered by tile parity.
+template <int K, int MODE, int SBITS, bool TABLE>
 __global__ __launch_bounds__((64 * fused_waves<MODE, K>())) void k_fused(
     const FusedIo io, const DeviceTables *__restrict__ tab)
 {
+  static_assert(!TABLE || SBITS == 0, "table-driven launches take int16 vectors");
   constexpr int P = kTileInst, W = fused_waves<MODE, K>(), R = fused_rows<MODE, K>();
+  __shared__ const int16_t *s_ip[TABLE ? 2 : 1][TABLE ? P : 1][8];
   __shared__ __align__(16) int16_t s_slab[W][P * kStride];
   __shared__ __align__(16) uint32_t s_g[K][P][64];  // (gamma*b1, (q-gamma)*b3) per group
   __shared__ __align__(16) int32_t s_bf2[16][16];
@@ -187,14 +209,33 @@ __global__ __launch_bounds__((64 * fused_waves<MODE, K>())) void k_fused(
 
   int16_t *const slab = s_slab[w];
   int16_t *const pslab = slab + tt * kStride;
-  // Matrix row of output row w for instance inst (K consecutive polys).
-  auto row_base = [&](size_t inst) -> const int16_t * {
+  // Buffer of a role for tile instance pp (i0 and the parity of the tile
+  // are the loop's): the table's pointer, or base + instance * stride.
+  size_t i0 = 0;
+  int par = 0;
+  auto ip = [&](int role, int pp) -> int16_t * {
+    if constexpr (TABLE) {
+      return const_cast<int16_t *>(s_ip[par][pp][role]);
+    } else {
+      const int16_t *base = role == rOut0   ? io.out0
+                            : role == rOut1 ? io.out1
+                            : role == rMat  ? io.mat
+                            : role == rMat2 ? io.mat2
+                            : role == rX    ? static_cast<const int16_t *>(io.x)
+                            : role == rAdd0 ? io.add0
+                            : role == rAdd1 ? io.add1
+                                            : io.add2;
+      return base ? const_cast<int16_t *>(base) + (i0 + pp) * role_stride<MODE, K>(role) : nullptr;
+    }
+  };
+  // Matrix row of output row w for tile instance pp (K consecutive polys).
+  auto row_base = [&](int pp) -> const int16_t * {
     if constexpr (MODE == kFusedEncrypt)
-      return w < K ? io.mat + (inst * K + w) * (size_t)(K * kN) : io.mat2 + inst * K * kN;
+      return w < K ? ip(rMat, pp) + (size_t)w * K * kN : ip(rMat2, pp);
     else if constexpr (MODE == kFusedDecrypt)
-      return io.mat + inst * K * kN;
+      return ip(rMat, pp);
     else
-      return io.mat + (inst * K + w) * (size_t)(K * kN);
+      return ip(rMat, pp) + (size_t)w * K * kN;
   };
   // Forward transform of the staged slab to canonical values.
   auto forward = [&]() {
@@ -206,17 +247,28 @@ __global__ __launch_bounds__((64 * fused_waves<MODE, K>())) void k_fused(
   };
 
   const size_t ntiles = (io.ninst + P - 1) / P;
-  for (size_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    const size_t i0 = tile * P;
+  for (size_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x, par ^= (TABLE ? 1 : 0)) {
+    i0 = tile * P;
     const int nvalid = (int)(io.ninst - i0 < (size_t)P ? io.ninst - i0 : P);
+    if constexpr (TABLE) {
+      // Buffer par was last read in the tile before the previous one, whose
+      // readers have all passed the previous tile's barriers.
+      if (w == 0)
+        for (int idx = lane; idx < P * 8; idx += 64) {
+          const int pp = idx >> 3, role = idx & 7;
+          s_ip[par][pp][role] =
+              pp < nvalid ? reinterpret_cast<const int16_t *const *>(io.inst + i0 + pp)[role]
+                          : nullptr;
+        }
+      __syncthreads();
+    }
 
     // 1. x_hat[w] = ntt(x_w), canonical, and its gamma products.
     if (w < K) {
       if constexpr (SBITS == 0) {
         tile_in<P>(slab,
                    [&](int pp) -> const int16_t * {
-                     return pp < nvalid ? static_cast<const int16_t *>(io.x) + ((i0 + pp) * K + w) * kN
-                                        : nullptr;
+                     return pp < nvalid ? ip(rX, pp) + (size_t)w * kN : nullptr;
                    },
                    lane);
       } else {
@@ -242,7 +294,7 @@ __global__ __launch_bounds__((64 * fused_waves<MODE, K>())) void k_fused(
           const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
           const v4i v = *reinterpret_cast<const v4i *>(slab + pp * kStride + ch * 8);
           if (pp < nvalid)
-            st_stream(reinterpret_cast<v4i *>(io.out1 + ((i0 + pp) * K + w) * kN) + ch,
+            st_stream(reinterpret_cast<v4i *>(ip(rOut1, pp) + (size_t)w * kN) + ch,
                       barrett_v4(v));
         }
       }
@@ -256,7 +308,7 @@ __global__ __launch_bounds__((64 * fused_waves<MODE, K>())) void k_fused(
     if (w < R) {
       auto load_row = [&](v4i *a, int it) {
         const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
-        const v4i *arow = reinterpret_cast<const v4i *>(row_base(i0 + pp)) + ch;
+        const v4i *arow = reinterpret_cast<const v4i *>(row_base(pp)) + ch;
 #pragma unroll
         for (int j = 0; j < K; j++)
           a[j] = pp < nvalid ? ld_stream(arow + j * (kN / 8)) : v4i{0, 0, 0, 0};
@@ -300,7 +352,7 @@ __global__ __launch_bounds__((64 * fused_waves<MODE, K>())) void k_fused(
         // t_hat_w = reduce(row + poly_ntt(e_w)), NTT domain.
         tile_in<P>(slab,
                    [&](int pp) -> const int16_t * {
-                     return pp < nvalid ? io.add0 + ((i0 + pp) * K + w) * kN : nullptr;
+                     return pp < nvalid ? ip(rAdd0, pp) + (size_t)w * kN : nullptr;
                    },
                    lane);
         forward();
@@ -312,7 +364,7 @@ __global__ __launch_bounds__((64 * fused_waves<MODE, K>())) void k_fused(
                                     (int)res[it][3]},
                                 e);  // < 3q: no wrap
           if (pp < nvalid)
-            st_stream(reinterpret_cast<v4i *>(io.out0 + ((i0 + pp) * K + w) * kN) + ch,
+            st_stream(reinterpret_cast<v4i *>(ip(rOut0, pp) + (size_t)w * kN) + ch,
                       barrett_v4(sum));
         }
       } else {
@@ -330,7 +382,7 @@ __global__ __launch_bounds__((64 * fused_waves<MODE, K>())) void k_fused(
         if constexpr (MODE == kFusedMatvec) {
           tile_out<P>(slab,
                       [&](int pp) -> int16_t * {
-                        return pp < nvalid ? io.out0 + ((i0 + pp) * K + w) * kN : nullptr;
+                        return pp < nvalid ? ip(rOut0, pp) + (size_t)w * kN : nullptr;
                       },
                       lane);
         } else {
@@ -339,26 +391,24 @@ __global__ __launch_bounds__((64 * fused_waves<MODE, K>())) void k_fused(
             const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
             if (pp >= nvalid)
               continue;
-            const size_t inst = i0 + pp;
             const v4i x = *reinterpret_cast<const v4i *>(slab + pp * kStride + ch * 8);
             const v4i zero = {0, 0, 0, 0};
             if constexpr (MODE == kFusedEncrypt) {
               if (w < K) {  // u_w = reduce(x + e1_w)
                 const v4i e1 =
-                    ld_stream(reinterpret_cast<const v4i *>(io.add0 + (inst * K + w) * kN) + ch);
-                st_stream(reinterpret_cast<v4i *>(io.out0 + (inst * K + w) * kN) + ch,
+                    ld_stream(reinterpret_cast<const v4i *>(ip(rAdd0, pp) + (size_t)w * kN) + ch);
+                st_stream(reinterpret_cast<v4i *>(ip(rOut0, pp) + (size_t)w * kN) + ch,
                           centred_sum<1>(x, e1, zero));
               } else {  // v = reduce(x + e2 + m)
-                const v4i e2 = ld_stream(reinterpret_cast<const v4i *>(io.add1 + inst * kN) + ch);
-                const v4i m = io.add2
-                                  ? ld_stream(reinterpret_cast<const v4i *>(io.add2 + inst * kN) + ch)
-                                  : zero;
-                st_stream(reinterpret_cast<v4i *>(io.out1 + inst * kN) + ch,
+                const v4i e2 = ld_stream(reinterpret_cast<const v4i *>(ip(rAdd1, pp)) + ch);
+                const int16_t *mp = ip(rAdd2, pp);
+                const v4i m = mp ? ld_stream(reinterpret_cast<const v4i *>(mp) + ch) : zero;
+                st_stream(reinterpret_cast<v4i *>(ip(rOut1, pp)) + ch,
                           centred_sum<1>(x, e2, m));
               }
             } else {  // decrypt: w = reduce(v - x)
-              const v4i vin = ld_stream(reinterpret_cast<const v4i *>(io.add0 + inst * kN) + ch);
-              st_stream(reinterpret_cast<v4i *>(io.out0 + inst * kN) + ch,
+              const v4i vin = ld_stream(reinterpret_cast<const v4i *>(ip(rAdd0, pp)) + ch);
+              st_stream(reinterpret_cast<v4i *>(ip(rOut0, pp)) + ch,
                         centred_sum<-1>(x, vin, zero));
             }
           }
@@ -381,31 +431,42 @@ int launch_fused(int k, int mode, int s_bits, const FusedIo &io, const DeviceTab
     hipLaunchKernelGGL(kernel, grid, dim3(64 * waves), 0, stream, io, tab);
     return 0;
   };
-  auto with_k = [&](auto mode_c, auto bits_c) {
+  auto with_k = [&](auto mode_c, auto bits_c, auto table_c) {
     constexpr int M = decltype(mode_c)::value, B = decltype(bits_c)::value;
+    constexpr bool T = decltype(table_c)::value;
     switch (k) {
-      case 2: return go(k_fused<2, M, B>, fused_waves<M, 2>());
-      case 3: return go(k_fused<3, M, B>, fused_waves<M, 3>());
-      case 4: return go(k_fused<4, M, B>, fused_waves<M, 4>());
+      case 2: return go(k_fused<2, M, B, T>, fused_waves<M, 2>());
+      case 3: return go(k_fused<3, M, B, T>, fused_waves<M, 3>());
+      case 4: return go(k_fused<4, M, B, T>, fused_waves<M, 4>());
       default: return -1;
     }
   };
   using std::integral_constant;
+  using Lin = std::false_type;
+  using Tab = std::true_type;
+  auto by_table = [&](auto mode_c) {
+    return io.inst ? with_k(mode_c, integral_constant<int, 0>{}, Tab{})
+                   : with_k(mode_c, integral_constant<int, 0>{}, Lin{});
+  };
+  if (io.inst && s_bits)
+    return -1;
   switch (mode) {
-    case kFusedMatvec:
+    case kFusedMatvec: {
+      const integral_constant<int, kFusedMatvec> mc{};
       switch (s_bits) {
-        case 0: return with_k(integral_constant<int, kFusedMatvec>{}, integral_constant<int, 0>{});
-        case 2: return with_k(integral_constant<int, kFusedMatvec>{}, integral_constant<int, 2>{});
-        case 3: return with_k(integral_constant<int, kFusedMatvec>{}, integral_constant<int, 3>{});
-        case 4: return with_k(integral_constant<int, kFusedMatvec>{}, integral_constant<int, 4>{});
+        case 0: return by_table(mc);
+        case 2: return with_k(mc, integral_constant<int, 2>{}, Lin{});
+        case 3: return with_k(mc, integral_constant<int, 3>{}, Lin{});
+        case 4: return with_k(mc, integral_constant<int, 4>{}, Lin{});
         default: return -1;
       }
+    }
     case kFusedKeygen:
-      return s_bits ? -1 : with_k(integral_constant<int, kFusedKeygen>{}, integral_constant<int, 0>{});
+      return s_bits ? -1 : by_table(integral_constant<int, kFusedKeygen>{});
     case kFusedEncrypt:
-      return s_bits ? -1 : with_k(integral_constant<int, kFusedEncrypt>{}, integral_constant<int, 0>{});
+      return s_bits ? -1 : by_table(integral_constant<int, kFusedEncrypt>{});
     case kFusedDecrypt:
-      return s_bits ? -1 : with_k(integral_constant<int, kFusedDecrypt>{}, integral_constant<int, 0>{});
+      return s_bits ? -1 : by_table(integral_constant<int, kFusedDecrypt>{});
     default:
       return -1;
   }

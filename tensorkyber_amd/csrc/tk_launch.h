@@ -78,6 +78,22 @@ int launch_pack_small(int bits, uint8_t *out, const int16_t *in, size_t npoly, i
 //                  add2 = m [1] or null
 //   kFusedDecrypt: out0 = w [1] <- mat = s_hat [k], x = u [k], add0 = v [1]
 enum FusedMode { kFusedMatvec = 0, kFusedKeygen = 1, kFusedEncrypt = 2, kFusedDecrypt = 3 };
+
+// One instance's buffers for a table-driven launch, in FusedIo's roles: each
+// points at that instance's first poly of the role (the role's shape as
+// above); add2 may be null.  Eight pointers, read as an array by role.
+struct FusedInst {
+  int16_t *out0;
+  int16_t *out1;
+  const int16_t *mat;
+  const int16_t *mat2;
+  const int16_t *x;
+  const int16_t *add0;
+  const int16_t *add1;
+  const int16_t *add2;
+};
+static_assert(sizeof(FusedInst) == 8 * sizeof(void *), "eight pointers");
+
 struct FusedIo {
   int16_t *out0;
   int16_t *out1;
@@ -88,6 +104,9 @@ struct FusedIo {
   const int16_t *add1;
   const int16_t *add2;
   size_t ninst;
+  // Non-null: instance g's buffers are inst[g] (int16 x only) and the
+  // pointers above are unused.
+  const FusedInst *inst;
 };
 
 // s_bits = 0: x is int16 polys; 2, 3 or 4: x is packed (kFusedMatvec only).

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include "tk_internal.h"
+#include "tk_launch.h"
 
 namespace tk {
 
@@ -31,6 +32,11 @@ int device_form(int form);
 // invntt_batch_ex).
 int transform(bool inverse, int16_t *out, const int16_t *in, size_t npoly, int method, int form,
               void *stream);
+// Argument rules of a fused call (k, required buffers, alignment, the one
+// exact alias each output may have, no other overlap); TK_OK or TK_EINVAL.
+int validate_fused(int k, int mode, const FusedIo &io);
+// validate_fused, then one launch on stream.
+int run_fused_checked(int k, int mode, const FusedIo &io, void *stream);
 const HostMatrices &host_matrices();
 const DeviceTables &host_device_tables();
 

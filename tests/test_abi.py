@@ -131,6 +131,31 @@ def test_host_pipeline_validation(tk):
     assert L.ntt_batch_host(None, 0) == tk.OK
 
 
+def test_fused_plan_validation(tk):
+    L = tk.lib()
+    plan = ctypes.c_void_p()
+    assert L.tk_fused_plan_create(ctypes.byref(plan), None, 1) == tk.EINVAL
+    buf = (ctypes.c_int16 * (N * 64 + 16))()
+    al = (ctypes.addressof(buf) + 15) & ~15
+    reqs = (tk.TkFusedRequest * 2)()
+    reqs[0].mode, reqs[0].k, reqs[0].ninst = tk.FUSED_MATVEC, 3, 1
+    reqs[0].out0, reqs[0].mat, reqs[0].x = al, al + 3 * 512, al + 12 * 512
+    reqs[1].mode, reqs[1].k, reqs[1].ninst = tk.FUSED_DECRYPT, 2, 0  # empty: skipped
+    table = ctypes.addressof(reqs)
+    reqs[0].mode = 7
+    assert L.tk_fused_plan_create(ctypes.byref(plan), table, 2) == tk.EINVAL   # bad mode
+    reqs[0].mode, reqs[0].k = tk.FUSED_MATVEC, 5
+    assert L.tk_fused_plan_create(ctypes.byref(plan), table, 2) == tk.EINVAL   # bad rank
+    reqs[0].k, reqs[0].x = 3, None
+    assert L.tk_fused_plan_create(ctypes.byref(plan), table, 2) == tk.EINVAL   # missing x
+    reqs[0].x, reqs[0].out0 = al + 12 * 512, al + 4 * 512
+    assert L.tk_fused_plan_create(ctypes.byref(plan), table, 2) == tk.EINVAL   # out over mat
+    reqs[0].ninst = 0
+    assert L.tk_fused_plan_create(ctypes.byref(plan), table, 2) == tk.EINVAL   # nothing to do
+    assert L.tk_fused_plan_destroy(None) == tk.EINVAL
+    assert L.tk_fused_plan_execute(None, None) == tk.EINVAL
+
+
 def test_workload_determinism(tk):
     a = tk.wl_uniform(1 << 17, 42)
     b = tk.wl_uniform(1 << 17, 42)
