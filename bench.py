@@ -30,6 +30,9 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_POLY_PASS = 1024  # 512 B read + 512 B written per poly per transform
 
 
+EXTRAS = ("ragged", "matvec", "packed", "kpke", "sched", "host")
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -42,6 +45,10 @@ def parse():
     ap.add_argument("--only", default=None, help="time only this method (profiling runs)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the ragged-plan and fused-matvec sections")
+    ap.add_argument("--extras", default=",".join(EXTRAS),
+                    help="comma list of extra sections to run: " + ", ".join(EXTRAS))
+    ap.add_argument("--extras-only", action="store_true",
+                    help="profiling run: only the selected extra sections, printed as JSON")
     ap.add_argument("--kernels-only", action="store_true",
                     help="profiling run: launch each kernel `steps` times plus a same-size "
                          "device copy, print per-kernel times, no CPU leg or parity check")
@@ -474,6 +481,28 @@ def load_traffic(kernel, npoly):
         return None
 
 
+def run_extras(tk, torch, args):
+    """The sections beyond the headline, as selected by --extras."""
+    sections = {
+        "ragged": ("config4_ragged", "config 4 ragged plan", lambda: bench_ragged(tk, torch, args.steps)),
+        "matvec": ("config5_matvec", "config 5 fused matvec", lambda: bench_matvec(tk, torch, args.steps)),
+        "packed": ("packed_input", "packed input", lambda: bench_packed(tk, torch, args.steps)),
+        "kpke": ("kpke_cores", "K-PKE cores", lambda: bench_kpke(tk, torch, args.steps)),
+        "sched": ("fused_plan_mix", "rank-aware fused plan", lambda: bench_sched(tk, torch, args.steps)),
+        "host": ("host_pipeline", "host-resident batches (PCIe)", lambda: bench_host(tk, torch)),
+    }
+    want = [e.strip() for e in args.extras.split(",") if e.strip()]
+    unknown = [e for e in want if e not in sections]
+    if unknown:
+        raise SystemExit(f"unknown --extras {unknown}; choose from {list(sections)}")
+    out = {}
+    for name in want:
+        key, label, fn = sections[name]
+        print(f"[bench] {label}", file=sys.stderr, flush=True)
+        out[key] = fn()
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -489,6 +518,10 @@ def main():
     rc = tk.lib().tk_init()
     if rc != 0:
         raise SystemExit(f"tk_init failed: {rc}")
+
+    if args.extras_only:
+        print(json.dumps(run_extras(tk, torch, args)))
+        return
 
     npoly = args.npoly
     x = tk.workload(3, npoly)
@@ -529,20 +562,7 @@ def main():
         torch.cuda.synchronize()
         bit_exact = check_parity(tk, torch, main_m, x, fwd, rt)
 
-    extras = {}
-    if not args.no_extras:
-        print("[bench] config 4 ragged plan", file=sys.stderr, flush=True)
-        extras["config4_ragged"] = bench_ragged(tk, torch, args.steps)
-        print("[bench] config 5 fused matvec", file=sys.stderr, flush=True)
-        extras["config5_matvec"] = bench_matvec(tk, torch, args.steps)
-        print("[bench] packed input", file=sys.stderr, flush=True)
-        extras["packed_input"] = bench_packed(tk, torch, args.steps)
-        print("[bench] K-PKE cores", file=sys.stderr, flush=True)
-        extras["kpke_cores"] = bench_kpke(tk, torch, args.steps)
-        print("[bench] rank-aware fused plan", file=sys.stderr, flush=True)
-        extras["fused_plan_mix"] = bench_sched(tk, torch, args.steps)
-        print("[bench] host-resident batches (PCIe)", file=sys.stderr, flush=True)
-        extras["host_pipeline"] = bench_host(tk, torch)
+    extras = {} if args.no_extras else run_extras(tk, torch, args)
 
     cpu = None
     if not args.no_cpu:
