@@ -24,8 +24,9 @@
 //   encrypt: k+1 waves and rows: u_i from row i of the transposed matrix,
 //            v from t_hat; step 3 inverse-transforms and adds e1_i, or e2 and
 //            m;
-//   decrypt: one row s_hat . u_hat; step 3 inverse-transforms and writes
-//            w = reduce(v - row).
+//   decrypt: one row s_hat . u_hat, formed as k partial rows (wave j:
+//            s_hat_j o u_hat_j) summed by wave 0; step 3 inverse-transforms
+//            and writes w = reduce(v - row).
 // The adds are exact mod q.  The reference's poly_add / poly_sub are int16
 // adds on invntt's (-q, q) representatives; they agree with these wherever
 // they do not overflow int16, which holds for every ML-KEM input (noise of
@@ -93,7 +94,7 @@ __device__ __forceinline__ v4i barrett_v4(const v4i x)
   return r;
 }
 
-// Lane-wise int16 add (no wrap for the operands it is used on: < 3q).  The
+// Lane-wise int16 add (no wrap for the operands it is used on: < 8q).  The
 // elements are copied to scalars first: __builtin_bit_cast of a vector
 // element lvalue reads element 0 here, whatever the index.
 __device__ __forceinline__ v4i add16(const v4i a, const v4i b)
@@ -305,7 +306,44 @@ __global__ __launch_bounds__((64 * fused_waves<MODE, K>())) void k_fused(
     // matrix loads of the next chunk are in flight while this one is summed;
     // the scheduling barrier keeps the compiler from hoisting more of them.
     uint32_t res[P / 2][4];
-    if (w < R) {
+    if constexpr (MODE == kFusedDecrypt) {
+      // The one output row is split across the waves: wave w forms the
+      // partial s_hat_w o u_hat_w against its own slab and then overwrites
+      // that slab with it (no other wave reads it); wave 0 sums the partials
+      // after the barrier.
+      auto load_part = [&](v4i &a, int it) {
+        const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
+        a = pp < nvalid ? ld_stream(reinterpret_cast<const v4i *>(ip(rMat, pp) + (size_t)w * kN) + ch)
+                        : v4i{0, 0, 0, 0};
+      };
+      v4i a_cur, a_nxt;
+      load_part(a_cur, 0);
+#pragma unroll
+      for (int it = 0; it < P / 2; it++) {
+        const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
+        if (it + 1 < P / 2)
+          load_part(a_nxt, it + 1);
+        const v4i b = *reinterpret_cast<const v4i *>(slab + pp * kStride + ch * 8);
+        const uint2 gg = *reinterpret_cast<const uint2 *>(&s_g[w][pp][2 * ch]);
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const uint32_t bu = (uint32_t)b[u], gs = u < 2 ? gg.x : gg.y;
+          const uint32_t x = __builtin_amdgcn_perm(gs, bu, (u & 1) ? 0x07060100u : 0x05040100u);
+          const uint32_t y = __builtin_amdgcn_alignbit(bu, bu, 16);
+          res[it][u] = pack2(red32((uint32_t)dot2((uint32_t)a_cur[u], x, 0) + kSumOffset),
+                             red32((uint32_t)dot2((uint32_t)a_cur[u], y, 0) + kSumOffset));
+        }
+        a_cur = a_nxt;
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      wave_sync();  // this wave's reads of its slab are done
+#pragma unroll
+      for (int it = 0; it < P / 2; it++) {
+        const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
+        *reinterpret_cast<v4i *>(slab + pp * kStride + ch * 8) =
+            v4i{(int)res[it][0], (int)res[it][1], (int)res[it][2], (int)res[it][3]};
+      }
+    } else if (w < R) {
       auto load_row = [&](v4i *a, int it) {
         const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
         const v4i *arow = reinterpret_cast<const v4i *>(row_base(pp)) + ch;
@@ -345,6 +383,26 @@ __global__ __launch_bounds__((64 * fused_waves<MODE, K>())) void k_fused(
       }
     }
     __syncthreads();  // every wave is done reading the slabs
+
+    if constexpr (MODE == kFusedDecrypt) {
+      // Wave 0 sums the partials (each in [0, 2q), the sum below 8q) into
+      // its slab; the barrier then frees the other slabs for the next tile.
+      if (w == 0) {
+#pragma unroll
+        for (int it = 0; it < P / 2; it++) {
+          const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
+          v4i acc = *reinterpret_cast<const v4i *>(s_slab[0] + pp * kStride + ch * 8);
+#pragma unroll
+          for (int j = 1; j < K; j++)
+            acc = add16(acc, *reinterpret_cast<const v4i *>(s_slab[j] + pp * kStride + ch * 8));
+          res[it][0] = (uint32_t)acc[0];
+          res[it][1] = (uint32_t)acc[1];
+          res[it][2] = (uint32_t)acc[2];
+          res[it][3] = (uint32_t)acc[3];
+        }
+      }
+      __syncthreads();
+    }
 
     // 3. Output row w.
     if (w < R) {
