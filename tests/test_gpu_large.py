@@ -1,0 +1,115 @@
+"""Largest sizes and concurrent callers.
+
+* 2^23 polys (4 GiB per buffer): every method's forward and inverse, with
+  rows sampled across the whole range (both ends, around 2^22, random)
+  checked against the oracle, and the round trip invntt(ntt(x)) == 2^16 x
+  (mod q) checked on every coefficient, chunk by chunk, on the device.
+* Several host threads calling the library at once, each on its own stream
+  and buffers, through the transforms, the fused product and a ragged plan.
+"""
+import ctypes
+import threading
+
+import numpy as np
+import pytest
+
+from conftest import canon, centred
+
+pytestmark = pytest.mark.gpu
+
+N, Q = 256, 3329
+
+
+@pytest.mark.parametrize("method", ["mfma16", "mfma32", "butterfly"])
+def test_four_gib_batch(tk, gpu, oracle, method):
+    import torch
+    npoly = 1 << 23
+    g = torch.Generator(device=gpu)
+    g.manual_seed(23)
+    x = torch.randint(0, Q, (npoly, N), dtype=torch.int16, device=gpu, generator=g)
+    print(f"4 GiB batch, {method}", flush=True)
+    y = torch.empty_like(x)
+    tk.ntt(x, out=y, method=method)
+    rng = np.random.default_rng(5)
+    rows = np.unique(np.concatenate([
+        [0, 1, 15, 16, (1 << 22) - 1, 1 << 22, (1 << 22) + 1, npoly - 17, npoly - 1],
+        rng.integers(0, npoly, 2000)]))
+    ridx = torch.from_numpy(rows).to(gpu)
+    xs = x.index_select(0, ridx).cpu().numpy()
+    ys = y.index_select(0, ridx).cpu().numpy()
+    assert np.array_equal(ys, canon(oracle.ntt(xs))), "forward rows"
+    tk.invntt(y, method=method)
+    zs = y.index_select(0, ridx).cpu().numpy()
+    assert np.array_equal(zs, centred(oracle.invntt(ys))), "inverse rows"
+    # every coefficient: invntt(ntt(x)) == 2^16 x (mod q)
+    step = 1 << 20
+    for c in range(0, npoly, step):
+        want = (x[c:c + step].to(torch.int32) * 65536) % Q
+        got = y[c:c + step].to(torch.int32) % Q
+        assert torch.equal(got, want), f"round trip, polys {c}.."
+    del x, y
+    torch.cuda.empty_cache()
+
+
+def test_concurrent_host_threads(tk, gpu, oracle):
+    import torch
+    nthreads, reps = 4, 6
+    errors = []
+    barrier = threading.Barrier(nthreads)
+
+    def worker(tid):
+        try:
+            stream = torch.cuda.Stream(device=gpu)
+            k, ninst, npoly = 2 + tid % 3, 97 + tid, 4096 + 17 * tid
+            x = tk.wl_uniform(npoly * N, 700 + tid, 0, Q).reshape(npoly, N)
+            A = tk.wl_uniform(ninst * k * k * N, 710 + tid, 0, Q).reshape(ninst, k, k, N)
+            s = tk.wl_cbd2(ninst * k * N, 720 + tid).reshape(ninst, k, N)
+            want_f = canon(oracle.ntt(x))
+            want_t = centred(oracle.matvec(A, s, k)).reshape(ninst * k, N)
+            reqs, pool = tk.wl_ragged_layout(3000, 730 + tid)
+            rp = tk.wl_uniform(pool * N, 740 + tid, 0, Q).reshape(pool, N)
+            want_r = rp.copy()
+            for off, n, _k, _kind in reqs:
+                want_r[off:off + n] = canon(oracle.ntt(rp[off:off + n]))
+            with torch.cuda.stream(stream):
+                dx = torch.from_numpy(x).to(gpu, non_blocking=False)
+                dA = torch.from_numpy(A).to(gpu)
+                ds = torch.from_numpy(s).to(gpu)
+                drp = torch.from_numpy(rp).to(gpu)
+                src = drp.clone()
+                table = (tk.TkRequest * len(reqs))()
+                for r, (off, n, kk, kind) in enumerate(reqs):
+                    table[r].base = drp.data_ptr() + int(off) * N * 2
+                    table[r].npoly, table[r].k, table[r].kind = int(n), int(kk), int(kind)
+                plan = ctypes.c_void_p()
+                sp = ctypes.c_void_p(stream.cuda_stream)
+                rc = tk.lib().tk_ragged_plan_create(ctypes.byref(plan), ctypes.addressof(table),
+                                                    len(reqs), 0, tk.METHOD_MFMA16, sp)
+                assert rc == 0, f"thread {tid}: plan create {rc}"
+                barrier.wait()
+                for rep in range(reps):
+                    out = torch.empty_like(dx)
+                    tk.ntt(dx, out=out, stream=stream)
+                    t = tk.matvec(dA, ds, k, stream=stream)
+                    drp.copy_(src)
+                    assert tk.lib().tk_ragged_plan_execute(plan, sp) == 0
+                    stream.synchronize()
+                    assert np.array_equal(out.cpu().numpy(), want_f), f"thread {tid} rep {rep} ntt"
+                    assert np.array_equal(t.cpu().numpy().reshape(ninst * k, N), want_t), \
+                        f"thread {tid} rep {rep} matvec"
+                    assert np.array_equal(drp.cpu().numpy(), want_r), f"thread {tid} rep {rep} plan"
+                tk.lib().tk_ragged_plan_destroy(plan)
+        except Exception as e:  # noqa: BLE001  (reported below with the thread id)
+            errors.append(f"thread {tid}: {e!r}")
+            try:
+                barrier.abort()
+            except threading.BrokenBarrierError:
+                pass
+
+    threads = [threading.Thread(target=worker, args=(i,)) for i in range(nthreads)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=120)
+    assert not any(t.is_alive() for t in threads), "a worker did not finish"
+    assert not errors, errors
