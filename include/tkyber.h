@@ -304,7 +304,10 @@ int tk_host_tables(int16_t *zetas, int16_t *fwd1, int16_t *fwd2, int16_t *inv2, 
  * ops = [nmat][2 limbs][64 lanes][2 dwords]) or 32 (32x32x32 tiles,
  * ops = [nmat][64 lanes][4 dwords]) of stage 0 (forward layers 1-4, nmat 1),
  * 1 (forward layers 5-7, nmat 16), 2 (inverse layers 7-5, nmat 16) or 3
- * (inverse layers 4-1, nmat 1), and the row biases bias = [nmat][16]. */
+ * (inverse layers 4-1, nmat 1), and the row biases bias = [nmat][16].
+ * Stages 4, 5, 6 (shape 16 only, nmat 1): forward layers 1-4 for packed
+ * 2-, 3-, 4-bit input, whose data digits are the unsigned field f and its
+ * sign bit s (x = f - 2^bits s). */
 int tk_host_mfma_operands(int shape, int stage, uint32_t *ops, int32_t *bias);
 
 #ifdef __cplusplus

@@ -517,9 +517,18 @@ int tk_host_tables(int16_t *zetas, int16_t *fwd1, int16_t *fwd2, int16_t *inv2, 
 
 int tk_host_mfma_operands(int shape, int stage, uint32_t *ops, int32_t *bias)
 {
-  if ((shape != 16 && shape != 32) || stage < kFwd1 || stage > kInv1)
+  if ((shape != 16 && shape != 32) || stage < kFwd1 || stage > kInv1 + 3)
+    return TK_EINVAL;
+  if (stage > kInv1 && shape != 16)
     return TK_EINVAL;
   const DeviceTables &dt = host_device_tables();
+  if (stage > kInv1) {  // packed first pass, 2 + (stage - 4) bits
+    if (ops)
+      memcpy(ops, dt.m16_fwd1pk[stage - kInv1 - 1], sizeof(dt.m16_fwd1pk[0]));
+    if (bias)
+      memcpy(bias, dt.bias_fwd1pk, sizeof(dt.bias_fwd1pk));
+    return TK_OK;
+  }
   const void *src = nullptr;
   size_t n = 0;
   const int32_t *b = nullptr;

@@ -189,6 +189,12 @@ __global__ __launch_bounds__((64 * fused_waves<MODE, K>())) void k_fused(
   const long f1lo = ld_a(tab->m16_fwd1[0], 0, lane), f1hi = ld_a(tab->m16_fwd1[0], 1, lane);
   const long i1lo = ld_a(tab->m16_inv1p[0], 0, lane), i1hi = ld_a(tab->m16_inv1p[0], 1, lane);
   const v4i fb1 = *reinterpret_cast<const v4i *>(&tab->bias_fwd1[0][4 * g]);
+  // Packed secrets run the first pass straight from their (field, sign bit)
+  // digits (pass_m16_packed).
+  constexpr int kPk = SBITS ? SBITS - 2 : 0;
+  const long f1pklo = SBITS ? ld_a(tab->m16_fwd1pk[kPk], 0, lane) : 0;
+  const long f1pkhi = SBITS ? ld_a(tab->m16_fwd1pk[kPk], 1, lane) : 0;
+  const v4i fb1pk = *reinterpret_cast<const v4i *>(&tab->bias_fwd1pk[0][4 * g]);
   const v4i ib1 = *reinterpret_cast<const v4i *>(&tab->bias_inv1p[0][4 * g]);
   // The per-block matrices of both directions are fetched at each use from
   // the (cache-resident) table through a buffer descriptor: one lane-offset
@@ -272,17 +278,22 @@ __global__ __launch_bounds__((64 * fused_waves<MODE, K>())) void k_fused(
                      return pp < nvalid ? ip(rX, pp) + (size_t)w * kN : nullptr;
                    },
                    lane);
+        forward();
       } else {
         constexpr size_t PPB = packed_poly_bytes<SBITS>();
-        tile_in_packed<P, SBITS>(slab,
+        uint32_t pv[packed_lane_dwords<SBITS>()];
+        packed_load_lanes<SBITS>(pv,
                                  [&](int pp) -> const uint8_t * {
                                    return pp < nvalid ? static_cast<const uint8_t *>(io.x) +
                                                             ((i0 + pp) * K + w) * PPB
                                                       : nullptr;
                                  },
                                  lane);
+        pass_m16_packed<SBITS>(pslab, g, pv, f1pklo, f1pkhi, fb1pk);
+        wave_sync();
+        pass_c16<kFormCanon>(pslab, g, f2lo, f2hi, s_bf2);
+        wave_sync();
       }
-      forward();
 #pragma unroll 4
       for (int pp = 0; pp < P; pp++) {
         const uint2 d = ld2(slab + pp * kStride + 4 * lane);

@@ -57,6 +57,10 @@ struct DeviceTables {
   // Inverse layers 4-1 without the Montgomery factor: plain NTT^-1 =
   // inv1 * 2^-16, used where the input carries no 2^-16 (fused products).
   uint32_t m16_inv1p[1][2][64][2];
+  // Forward layers 1-4 for packed b-bit input (b = 2, 3, 4 at [b-2]): the
+  // digits are the unsigned field f and its sign bit s, x = f - 2^b s, so
+  // V = [ W | -2^b W mod q ]; the row bias carries no 128 offset.
+  uint32_t m16_fwd1pk[3][2][64][2];
   // MFMA32 operands: [matrix][lane][4 dwords]
   uint32_t m32_fwd1[1][64][4];
   uint32_t m32_fwd2[16][64][4];
@@ -68,6 +72,7 @@ struct DeviceTables {
   int32_t bias_inv2[16][16];
   int32_t bias_inv1[1][16];
   int32_t bias_inv1p[1][16];  // includes kCentre
+  int32_t bias_fwd1pk[1][16];  // kBiasQ * q only
   // Plain-domain basemul twiddles gamma_i = zetas[64+i] * 2^-16 mod q in [0, q)
   // (pair (4i,4i+1) uses gamma_i, pair (4i+2,4i+3) uses q - gamma_i).
   uint16_t gamma[64];

@@ -15,6 +15,8 @@
 // VGPRs for the life of the wave.  Each wave owns an LDS slab holding its
 // tile; the two passes read and write only lane-private LDS words, so the only
 // cross-lane hand-offs are tile load -> pass 1 -> pass 2 -> tile store.
+#include <type_traits>
+
 #include "tk_launch.h"
 #include "tk_mfma_core.h"
 
@@ -148,17 +150,24 @@ __device__ __forceinline__ void stage_load(const Tiles &, size_t, StageRegs<P, T
 {
 }
 
+// Packed tiles of 16 polys (MFMA16) are loaded in the lane layout of the
+// direct first pass (lane (g, t): coefficients 64g .. 64g+63 of poly t) and
+// never unpacked; 32-poly tiles (MFMA32) are unpacked into the slab.
 template <int P, int BITS>
 __device__ __forceinline__ void stage_load(const PackedTiles<BITS> &io, size_t tile,
                                            StageRegs<P, PackedTiles<BITS>> &r, int lane)
 {
   constexpr size_t PPB = packed_poly_bytes<BITS>();
-  packed_load<P, BITS>(r.v,
-                       [&](int pp) -> const uint8_t * {
-                         const size_t p = tile * P + pp;
-                         return p < io.npoly ? io.in + p * PPB : nullptr;
-                       },
-                       lane);
+  auto src = [&](int pp) -> const uint8_t * {
+    const size_t p = tile * P + pp;
+    return p < io.npoly ? io.in + p * PPB : nullptr;
+  };
+  if constexpr (P == 16) {
+    static_assert(packed_regs<P, BITS>() == packed_lane_dwords<BITS>(), "one layout per lane");
+    packed_load_lanes<BITS>(r.v, src, lane);
+  } else {
+    packed_load<P, BITS>(r.v, src, lane);
+  }
 }
 
 // Stage tile `tile` into the slab L: int16 polys through their parked
@@ -175,8 +184,17 @@ __device__ __forceinline__ void stage_in(const PackedTiles<BITS> &, size_t, int1
                                          int16_t *const *, const StageRegs<P, PackedTiles<BITS>> &r,
                                          int lane)
 {
-  packed_park<P, BITS>(L, r.v, lane);
+  if constexpr (P != 16)
+    packed_park<P, BITS>(L, r.v, lane);
 }
+
+// Bits per coefficient of packed tiles, 0 for int16 tiles.
+template <class Tiles>
+struct PackedBits : std::integral_constant<int, 0> {
+};
+template <int BITS>
+struct PackedBits<PackedTiles<BITS>> : std::integral_constant<int, BITS> {
+};
 
 template <int FORM, bool INV, int WAVES, class Tiles>
 __global__ __launch_bounds__(64 * WAVES) void k_mfma16(Tiles io,
@@ -193,11 +211,13 @@ __global__ __launch_bounds__(64 * WAVES) void k_mfma16(Tiles io,
     s_bias[i >> 4][i & 15] = bias_c[i >> 4][i & 15] + (INV ? 0 : bias_adjust<FORM, INV>());
   __syncthreads();
 
-  const auto &m1 = INV ? tab->m16_inv1[0] : tab->m16_fwd1[0];
+  // Packed input runs its first pass on (field, sign bit) digits.
+  constexpr int PB = PackedBits<Tiles>::value;
+  const auto &m1 = INV ? tab->m16_inv1[0] : (PB ? tab->m16_fwd1pk[PB ? PB - 2 : 0] : tab->m16_fwd1[0]);
   const auto &m2 = INV ? tab->m16_inv2 : tab->m16_fwd2;
   const long a1lo = *reinterpret_cast<const long *>(m1[0][lane]);
   const long a1hi = *reinterpret_cast<const long *>(m1[1][lane]);
-  const int32_t *b1p = INV ? tab->bias_inv1[0] : tab->bias_fwd1[0];
+  const int32_t *b1p = INV ? tab->bias_inv1[0] : (PB ? tab->bias_fwd1pk[0] : tab->bias_fwd1[0]);
   const int32_t b1adj = INV ? bias_adjust<FORM, INV>() : 0;
   const v4i bias1 = {b1p[4 * g] + b1adj, b1p[4 * g + 1] + b1adj, b1p[4 * g + 2] + b1adj,
                      b1p[4 * g + 3] + b1adj};
@@ -227,7 +247,10 @@ __global__ __launch_bounds__(64 * WAVES) void k_mfma16(Tiles io,
     stage_in<P>(io, tile, L, s_ptr[w], sr, lane);
     wave_sync();
     if (!INV) {
-      pass_m16<kFormInter>(Pp, g, a1lo, a1hi, bias1);
+      if constexpr (PB != 0)
+        pass_m16_packed<PB>(Pp, g, sr.v, a1lo, a1hi, bias1);
+      else
+        pass_m16<kFormInter>(Pp, g, a1lo, a1hi, bias1);
       wave_sync();
       pass_c16<FORM>(Pp, g, [&](int m) { return a2lo[m]; }, [&](int m) { return a2hi[m]; },
                      s_bias);

@@ -248,6 +248,93 @@ __device__ __forceinline__ void pass_m16(int16_t *P, int g, long alo, long ahi, 
   }
 }
 
+// First pass straight from packed small-coefficient input (MFMA16, forward
+// layers 1-4).  Lane (g, t) holds, in v, the packed bytes of coefficients
+// 64g .. 64g+63 of poly t -- rows m = 4g .. 4g+3 of x[m][c] -- as loaded by
+// packed_load_lanes: 4 dwords (one row each) for 2 bits, 8 (two per row)
+// for 3 or 4 bits.  The data digits are the unsigned field f and its sign
+// bit s (x = f - 2^BITS s; operands m16_fwd1pk): for a column c the four
+// rows' fields sit in one byte position of the four row words, so a byte
+// gather and a shift-and-mask give each digit of four rows at once -- no
+// unpacking to int16, no LDS round trip and no byte transpose.  Results go
+// to the slab as pass_m16 writes them.
+template <int BITS>
+constexpr int packed_lane_dwords()
+{
+  return packed_poly_bytes<BITS>() / 16;
+}
+
+template <int BITS, class Addr>
+__device__ __forceinline__ void packed_load_lanes(uint32_t *v, Addr src, int lane)
+{
+  constexpr int NDW = packed_lane_dwords<BITS>();
+  const int g = lane >> 4, t = lane & 15;
+  const uint8_t *p = src(t);
+#pragma unroll
+  for (int i = 0; i < NDW / 4; i++) {
+    const v4i x = p ? ld_stream(reinterpret_cast<const v4i *>(p + g * NDW * 4) + i)
+                    : v4i{0, 0, 0, 0};
+    v[4 * i] = (uint32_t)x[0];
+    v[4 * i + 1] = (uint32_t)x[1];
+    v[4 * i + 2] = (uint32_t)x[2];
+    v[4 * i + 3] = (uint32_t)x[3];
+  }
+}
+
+// Byte j of each of d0..d3, as bytes 0..3.
+__device__ __forceinline__ uint32_t gather_byte(uint32_t d0, uint32_t d1, uint32_t d2, uint32_t d3,
+                                                int j)
+{
+  const uint32_t lo = __builtin_amdgcn_perm(d1, d0, 0x0C0C0000u | ((4u + j) << 8) | (uint32_t)j);
+  const uint32_t hi = __builtin_amdgcn_perm(d3, d2, ((4u + j) << 24) | ((uint32_t)j << 16) | 0x0C0Cu);
+  return lo | hi;
+}
+
+template <int BITS, int G = kPassGroup>
+__device__ __forceinline__ void pass_m16_packed(int16_t *P, int g, const uint32_t *v, long alo,
+                                                long ahi, v4i bias)
+{
+  static_assert(G == 4, "four columns per group");
+  constexpr uint32_t kFieldMask = ((1u << BITS) - 1u) * 0x01010101u;
+  const v4i zero = {0, 0, 0, 0};
+  int16_t *rows = P + 64 * g;  // row 4g; row 4g + r at +16r
+#pragma unroll
+  for (int cq = 0; cq < 4; cq++) {
+    long b[4];
+#pragma unroll
+    for (int h = 0; h < 4; h++) {
+      uint32_t gb;
+      int sh;
+      if (BITS == 2) {  // row r = dword r; column c at byte c/4, bit 2(c%4)
+        gb = gather_byte(v[0], v[1], v[2], v[3], cq);
+        sh = 2 * h;
+      } else {  // row r = dwords 2r, 2r+1; column c at row byte c/2, bit BITS*(c%2)
+        const int rb = 2 * cq + (h >> 1), dw = rb >> 2, by = rb & 3;
+        gb = gather_byte(v[dw], v[2 + dw], v[4 + dw], v[6 + dw], by);
+        sh = BITS * (h & 1);
+      }
+      const uint32_t f = (gb >> sh) & kFieldMask;
+      const uint32_t sgn = (gb >> (sh + BITS - 1)) & 0x01010101u;
+      b[h] = (long)(((uint64_t)sgn << 32) | f);
+    }
+    v4i lo[4], hi[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      lo[u] = mfma16(alo, b[u], bias);
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      hi[u] = mfma16(ahi, b[u], zero);
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      uint32_t val[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        val[u] = reduce_2q(lo[u][r], hi[u][r]);
+      st2(rows + 16 * r + 4 * cq, pack2(val[0], val[1]), pack2(val[2], val[3]));
+    }
+  }
+}
+
 // Contraction over c inside each block m (one matrix per block), G blocks
 // per group.
 template <int MODE, class ALo, class AHi, int G = kPassGroup>

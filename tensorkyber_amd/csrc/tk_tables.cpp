@@ -92,20 +92,22 @@ void limbs(int v, int8_t *v0, int8_t *v1)
   *v1 = (int8_t)((v - lo) / 256);
 }
 
-// V[i][d][u]: d = 0 -> W[i][u], d = 1 -> 256*W[i][u] mod q; centred.
-void digit_matrix(const uint16_t W[16][16], int V[16][2][16])
+// V[i][d][u]: d = 0 -> W[i][u], d = 1 -> w1*W[i][u] mod q; centred.  w1 is
+// the weight of the second data digit: 256 for the byte digits of any int16,
+// -2^b for the (field, sign bit) digits of packed b-bit coefficients.
+void digit_matrix(const uint16_t W[16][16], int V[16][2][16], int64_t w1 = 256)
 {
   for (int i = 0; i < 16; i++)
     for (int u = 0; u < 16; u++) {
       V[i][0][u] = centred(W[i][u]);
-      V[i][1][u] = centred(256 * (int64_t)W[i][u]);
+      V[i][1][u] = centred(w1 * (int64_t)W[i][u]);
     }
 }
 
-void pack16(const uint16_t W[16][16], uint32_t out[2][64][2])
+void pack16(const uint16_t W[16][16], uint32_t out[2][64][2], int64_t w1 = 256)
 {
   int V[16][2][16];
-  digit_matrix(W, V);
+  digit_matrix(W, V, w1);
   for (int lane = 0; lane < 64; lane++) {
     const int g = lane >> 4, i = lane & 15;
     uint8_t b0[8], b1[8];
@@ -138,13 +140,15 @@ void pack32(const uint16_t W[16][16], uint32_t out[64][4])
   }
 }
 
-void row_bias(const uint16_t W[16][16], int extra, int32_t out[16])
+// offset: the constant the digits leave out of every data value (128 for
+// the byte digits, 0 for packed digits), which contributes offset * sum_u W.
+void row_bias(const uint16_t W[16][16], int extra, int32_t out[16], int offset = 128)
 {
   for (int i = 0; i < 16; i++) {
     int64_t s = 0;
     for (int u = 0; u < 16; u++)
       s += W[i][u];
-    out[i] = (int32_t)(modq(128 * s) + (int64_t)kBiasQ * kQ + extra);
+    out[i] = (int32_t)(modq(offset * s) + (int64_t)kBiasQ * kQ + extra);
   }
 }
 
@@ -207,6 +211,10 @@ void build_tables(HostMatrices *hm, DeviceTables *dt)
   pack32(hm->fwd1, dt->m32_fwd1[0]);
   pack32(hm->inv1, dt->m32_inv1[0]);
   row_bias(hm->fwd1, 0, dt->bias_fwd1[0]);
+  // Packed b-bit input: digits (field f, sign bit s) with x = f - 2^b s.
+  for (int b = 2; b <= 4; b++)
+    pack16(hm->fwd1, dt->m16_fwd1pk[b - 2], -(int64_t)(1 << b));
+  row_bias(hm->fwd1, 0, dt->bias_fwd1pk[0], 0);
   row_bias(hm->inv1, kCentre, dt->bias_inv1[0]);
   for (int b = 0; b < 16; b++) {
     pack16(hm->fwd2[b], dt->m16_fwd2[b]);

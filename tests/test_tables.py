@@ -73,15 +73,16 @@ def sbytes(words):
         np.int64)
 
 
-def emu16(ops, bias, data):
-    """One 16x16x32 MFMA pair: ops[2][64][2], data[16 cols][16 u] -> S[16][16]."""
+def emu16(ops, bias, data, d01=None):
+    """One 16x16x32 MFMA pair: ops[2][64][2], data[16 cols][16 u] -> S[16][16]
+    (d01: the two data digits, default the byte digits of `data`)."""
     A = np.zeros((2, 16, 32), np.int64)
     for lane in range(64):
         g, i = lane >> 4, lane & 15
         for limb in range(2):
             A[limb, i, 8 * g:8 * g + 8] = sbytes(ops[limb, lane])
     B = np.zeros((32, 16), np.int64)
-    d0, d1 = digits(data)
+    d0, d1 = digits(data) if d01 is None else d01
     for lane in range(64):
         g, j = lane >> 4, lane & 15
         B[8 * g:8 * g + 4, j] = d0[j, 4 * g:4 * g + 4]
@@ -130,6 +131,23 @@ def test_mfma16_operands(tk, T, stage, name):
             S = emu16(ops[mat], bias[mat], data)
             assert S.min() >= 0 and S.max() < (1 << 24)
             assert mod_ok(S, Wm, data, extra)
+
+
+@pytest.mark.parametrize("bits", [2, 3, 4])
+def test_mfma16_packed_first_pass_operands(tk, T, bits):
+    """Stage 2+bits: forward layers 1-4 on packed digits (field f, sign bit s),
+    x = f - 2^bits s, over every value of the field and random tiles."""
+    ops, bias = tk.host_mfma_operands(16, 2 + bits)
+    lo, hi = -(1 << (bits - 1)), 1 << (bits - 1)
+    tiles = [tk.wl_uniform(256, 360 + bits + t, lo, hi).reshape(16, 16) for t in range(4)]
+    tiles.append(np.resize(np.arange(lo, hi), 256).reshape(16, 16))
+    for x in tiles:
+        field = np.asarray(x, dtype=np.int64) & ((1 << bits) - 1)
+        f, sgn = field, field >> (bits - 1)
+        assert np.array_equal(f - (sgn << bits), x)
+        S = emu16(ops[0], bias[0], x, (f, sgn))
+        assert S.min() >= 0 and S.max() < (1 << 24)
+        assert mod_ok(S, T["fwd1"], x)
 
 
 @pytest.mark.parametrize("stage,name", [(0, "fwd1"), (1, "fwd2"), (2, "inv2"), (3, "inv1")])
