@@ -180,7 +180,7 @@ int transform(bool inverse, int16_t *out, const int16_t *in, size_t npoly, int m
     rc = launch_butterfly(inverse, device_form(f), out, in, npoly, st->tables, st->ncu, s);
   else
     rc = launch_mfma(m == TK_METHOD_MFMA32 ? 32 : 16, inverse, device_form(f), out, in, npoly,
-                     st->tables, LaunchShape{st->ncu, 4, kDefaultOversub}, s);
+                     st->tables, LaunchShape{st->ncu, 4, kLinearOversub}, s);
   if (rc)
     return TK_ENOTSUP;
   return after_launch();
@@ -466,7 +466,7 @@ int ntt_batch_packed(int16_t *out, const uint8_t *packed, size_t npoly, int bits
   if ((rc = init_device(&st)))
     return rc;
   if (launch_mfma_packed(m == TK_METHOD_MFMA32 ? 32 : 16, bits, device_form(f), out, packed, npoly,
-                         st->tables, LaunchShape{st->ncu, 4, kDefaultOversub}, (hipStream_t)stream))
+                         st->tables, LaunchShape{st->ncu, 4, kLinearOversub}, (hipStream_t)stream))
     return TK_ENOTSUP;
   return after_launch();
 }

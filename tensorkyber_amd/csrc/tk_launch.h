@@ -39,8 +39,11 @@ struct PackedTiles {
 // multiple of the workgroups resident at once (from the occupancy
 // calculator).  Each wave strides over tiles; a factor above 1 lets the
 // dispatcher hand late workgroups to whichever CU frees up first, which
-// balances the load (4 measured best for the 16-poly tiles).
+// balances the load.  Linear and packed batches use 8 (measured 2% faster
+// than 4 for the 16-poly tiles, which compute their poly addresses from the
+// tile index); ragged plans start from 4 and let the autotuner choose.
 constexpr int kDefaultOversub = 4;
+constexpr int kLinearOversub = 8;
 struct LaunchShape {
   int ncu;
   int waves;

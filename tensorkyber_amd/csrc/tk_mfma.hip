@@ -36,13 +36,12 @@ __device__ __forceinline__ int32_t bias_adjust()
   return FORM == kFormCentred ? 0 : -kCentre;
 }
 
-// Per-tile poly addresses.  Lanes 0..P-1 hold one poly address each and park
-// them in LDS, where the staging loops read them back; keeping them out of
-// vector registers across the two passes saves about 40 VGPRs, which is one
-// more resident wave per SIMD.  Linear tiles compute the addresses from the
-// tile index.  Ragged tiles read them from the plan's address table one tile
-// ahead (tile_fetch), so the read is in flight while the current tile is
-// staged and never stalls the wave on its own.
+// Per-tile poly addresses.  Linear (and packed) tiles compute them from the
+// tile index at each use.  Ragged tiles take them from the plan's address
+// table: lanes 0..P-1 read one address each one tile ahead (tile_fetch), so
+// the read is in flight while the current tile is staged, and park them in
+// LDS, where the staging loops read them back; that keeps P addresses out of
+// vector registers across the two passes.
 template <int P>
 __device__ __forceinline__ int16_t *tile_fetch(const LinearTiles &, size_t, int)
 {
@@ -50,29 +49,25 @@ __device__ __forceinline__ int16_t *tile_fetch(const LinearTiles &, size_t, int)
 }
 
 template <int P>
-__device__ __forceinline__ void tile_prepare(const LinearTiles &io, size_t tile, int lane,
-                                             int16_t *, int16_t **lds)
+__device__ __forceinline__ void tile_prepare(const LinearTiles &, size_t, int, int16_t *,
+                                             int16_t **)
 {
-  if (lane < P) {
-    const size_t p = tile * P + lane;
-    lds[lane] = p < io.npoly ? const_cast<int16_t *>(io.in) + p * kN : nullptr;
-    lds[P + lane] = p < io.npoly ? io.out + p * kN : nullptr;
-  }
-  wave_sync();
 }
 
 template <int P>
-__device__ __forceinline__ const int16_t *tile_src(const LinearTiles &, size_t, int pp,
-                                                   int16_t *const *lds)
+__device__ __forceinline__ const int16_t *tile_src(const LinearTiles &io, size_t tile, int pp,
+                                                   int16_t *const *)
 {
-  return lds[pp];
+  const size_t p = tile * P + pp;
+  return p < io.npoly ? io.in + p * kN : nullptr;
 }
 
 template <int P>
-__device__ __forceinline__ int16_t *tile_dst(const LinearTiles &, size_t, int pp,
-                                             int16_t *const *lds)
+__device__ __forceinline__ int16_t *tile_dst(const LinearTiles &io, size_t tile, int pp,
+                                             int16_t *const *)
 {
-  return lds[P + pp];
+  const size_t p = tile * P + pp;
+  return p < io.npoly ? io.out + p * kN : nullptr;
 }
 
 template <int P>
@@ -114,21 +109,17 @@ __device__ __forceinline__ int16_t *tile_fetch(const PackedTiles<BITS> &, size_t
 }
 
 template <int P, int BITS>
-__device__ __forceinline__ void tile_prepare(const PackedTiles<BITS> &io, size_t tile, int lane,
-                                             int16_t *, int16_t **lds)
+__device__ __forceinline__ void tile_prepare(const PackedTiles<BITS> &, size_t, int, int16_t *,
+                                             int16_t **)
 {
-  if (lane < P) {
-    const size_t p = tile * P + lane;
-    lds[P + lane] = p < io.npoly ? io.out + p * kN : nullptr;
-  }
-  wave_sync();
 }
 
 template <int P, int BITS>
-__device__ __forceinline__ int16_t *tile_dst(const PackedTiles<BITS> &, size_t, int pp,
-                                             int16_t *const *lds)
+__device__ __forceinline__ int16_t *tile_dst(const PackedTiles<BITS> &io, size_t tile, int pp,
+                                             int16_t *const *)
 {
-  return lds[P + pp];
+  const size_t p = tile * P + pp;
+  return p < io.npoly ? io.out + p * kN : nullptr;
 }
 
 // Staging registers: packed tiles are small (4-16 dwords per lane), so the
@@ -233,8 +224,8 @@ __global__ __launch_bounds__(64 * WAVES) void k_mfma16(Tiles io,
   // Grid-stride over tiles.  Latency is hidden by the other resident waves
   // (three per SIMD at this register count) and by launching several times
   // more workgroups than are resident, which also balances load across CUs;
-  // a register prefetch of the next tile measured slower (it costs a wave of
-  // occupancy).
+  // a register prefetch of the whole next tile measured slower (it costs a
+  // wave of occupancy), and one of half a tile, which fits, no faster.
   const size_t ntiles = (io.npoly + P - 1) / P;
   const size_t stride = (size_t)gridDim.x * WAVES;
   size_t tile = (size_t)blockIdx.x * WAVES + w;
