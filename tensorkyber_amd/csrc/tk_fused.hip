@@ -55,7 +55,6 @@ namespace tk {
 namespace {
 
 constexpr int kTileInst = 16;  // instances per tile: the MFMA16 poly width
-constexpr int kBufferRsrcWord3 = 0x00020000;  // gfx9 buffer descriptor: 32-bit data format
 // Multiple of q above every |sum|: 4 * 32768 * (3328 + 6657) < kSumOffset.
 constexpr uint32_t kSumOffset = (4u * 32768u * (3328u + 6657u) / 3329u + 1u) * 3329u;
 

@@ -36,8 +36,9 @@ __device__ __forceinline__ int32_t bias_adjust()
   return FORM == kFormCentred ? 0 : -kCentre;
 }
 
-// Per-tile poly addresses.  Linear (and packed) tiles compute them from the
-// tile index at each use.  Ragged tiles take them from the plan's address
+// Per-tile poly addresses.  Linear (and packed) tiles need none: they are
+// staged through a buffer descriptor over the tile (stage_in, stage_out
+// below).  Ragged tiles take them from the plan's address
 // table: lanes 0..P-1 read one address each one tile ahead (tile_fetch), so
 // the read is in flight while the current tile is staged, and park them in
 // LDS, where the staging loops read them back; that keeps P addresses out of
@@ -52,22 +53,6 @@ template <int P>
 __device__ __forceinline__ void tile_prepare(const LinearTiles &, size_t, int, int16_t *,
                                              int16_t **)
 {
-}
-
-template <int P>
-__device__ __forceinline__ const int16_t *tile_src(const LinearTiles &io, size_t tile, int pp,
-                                                   int16_t *const *)
-{
-  const size_t p = tile * P + pp;
-  return p < io.npoly ? io.in + p * kN : nullptr;
-}
-
-template <int P>
-__device__ __forceinline__ int16_t *tile_dst(const LinearTiles &io, size_t tile, int pp,
-                                             int16_t *const *)
-{
-  const size_t p = tile * P + pp;
-  return p < io.npoly ? io.out + p * kN : nullptr;
 }
 
 template <int P>
@@ -114,14 +99,6 @@ __device__ __forceinline__ void tile_prepare(const PackedTiles<BITS> &, size_t, 
 {
 }
 
-template <int P, int BITS>
-__device__ __forceinline__ int16_t *tile_dst(const PackedTiles<BITS> &io, size_t tile, int pp,
-                                             int16_t *const *)
-{
-  const size_t p = tile * P + pp;
-  return p < io.npoly ? io.out + p * kN : nullptr;
-}
-
 // Staging registers: packed tiles are small (4-16 dwords per lane), so the
 // next tile's loads are issued into registers before the current tile's
 // stores.  The wait for those loads then does not include the stores'
@@ -161,11 +138,29 @@ __device__ __forceinline__ void stage_load(const PackedTiles<BITS> &io, size_t t
   }
 }
 
-// Stage tile `tile` into the slab L: int16 polys through their parked
-// addresses, packed polys from the staging registers.
-template <int P, class Tiles>
-__device__ __forceinline__ void stage_in(const Tiles &io, size_t tile, int16_t *L,
-                                         int16_t *const *lds, const StageRegs<P, Tiles> &, int lane)
+// Bytes of the polys of tile `tile` that lie in a batch of npoly.
+template <int P>
+__device__ __forceinline__ uint32_t tile_bytes(size_t tile, size_t npoly, uint32_t poly_bytes)
+{
+  const size_t left = npoly - tile * P;
+  return (uint32_t)(left < (size_t)P ? left : (size_t)P) * poly_bytes;
+}
+
+// Stage tile `tile` into the slab L: linear tiles through a buffer
+// descriptor over the tile, ragged polys through their parked addresses,
+// packed polys from the staging registers.
+template <int P>
+__device__ __forceinline__ void stage_in(const LinearTiles &io, size_t tile, int16_t *L,
+                                         int16_t *const *, const StageRegs<P, LinearTiles> &,
+                                         int lane)
+{
+  tile_in_rsrc<P>(L, span_rsrc(io.in + tile * P * kN, tile_bytes<P>(tile, io.npoly, 2 * kN)), lane);
+}
+
+template <int P>
+__device__ __forceinline__ void stage_in(const RaggedTiles &io, size_t tile, int16_t *L,
+                                         int16_t *const *lds, const StageRegs<P, RaggedTiles> &,
+                                         int lane)
 {
   tile_in<P>(L, [&](int pp) { return tile_src<P>(io, tile, pp, lds); }, lane);
 }
@@ -179,6 +174,30 @@ __device__ __forceinline__ void stage_in(const PackedTiles<BITS> &, size_t, int1
     packed_park<P, BITS>(L, r.v, lane);
 }
 
+// Write the slab of tile `tile` to its output polys.
+template <int P>
+__device__ __forceinline__ void stage_out(const LinearTiles &io, size_t tile, const int16_t *L,
+                                          int16_t *const *, int lane)
+{
+  tile_out_rsrc<P>(L, span_rsrc(io.out + tile * P * kN, tile_bytes<P>(tile, io.npoly, 2 * kN)),
+                   lane);
+}
+
+template <int P, int BITS>
+__device__ __forceinline__ void stage_out(const PackedTiles<BITS> &io, size_t tile,
+                                          const int16_t *L, int16_t *const *, int lane)
+{
+  tile_out_rsrc<P>(L, span_rsrc(io.out + tile * P * kN, tile_bytes<P>(tile, io.npoly, 2 * kN)),
+                   lane);
+}
+
+template <int P>
+__device__ __forceinline__ void stage_out(const RaggedTiles &io, size_t tile, const int16_t *L,
+                                          int16_t *const *lds, int lane)
+{
+  tile_out<P>(L, [&](int pp) { return tile_dst<P>(io, tile, pp, lds); }, lane);
+}
+
 // Bits per coefficient of packed tiles, 0 for int16 tiles.
 template <class Tiles>
 struct PackedBits : std::integral_constant<int, 0> {
@@ -187,15 +206,17 @@ template <int BITS>
 struct PackedBits<PackedTiles<BITS>> : std::integral_constant<int, BITS> {
 };
 
+// Three waves per SIMD (at most 168 VGPRs): without the bound the scheduler
+// trades a wave of occupancy for longer-lived staging registers.
 template <int FORM, bool INV, int WAVES, class Tiles>
-__global__ __launch_bounds__(64 * WAVES) void k_mfma16(Tiles io,
+__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(3))) void k_mfma16(Tiles io,
                                                         const DeviceTables *__restrict__ tab)
 {
   constexpr int P = 16;
   __shared__ __align__(16) int16_t s_poly[WAVES][P * kStride];
   __shared__ int16_t *s_ptr[WAVES][2 * P];
   __shared__ __align__(16) int32_t s_bias[16][16];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, g = lane >> 4, t = lane & 15;
+  const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), g = lane >> 4, t = lane & 15;
 
   const int32_t(*bias_c)[16] = INV ? tab->bias_inv2 : tab->bias_fwd2;
   for (int i = threadIdx.x; i < 256; i += 64 * WAVES)
@@ -253,20 +274,21 @@ __global__ __launch_bounds__(64 * WAVES) void k_mfma16(Tiles io,
     }
     wave_sync();
     stage_load<P>(io, tile + stride, sr, lane);
-    tile_out<P>(L, [&](int pp) { return tile_dst<P>(io, tile, pp, s_ptr[w]); }, lane);
+    stage_out<P>(io, tile, L, s_ptr[w], lane);
     wave_sync();
   }
 }
 
+// Two waves per SIMD (at most 256 VGPRs), as for k_mfma16.
 template <int FORM, bool INV, int WAVES, class Tiles>
-__global__ __launch_bounds__(64 * WAVES) void k_mfma32(Tiles io,
+__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2))) void k_mfma32(Tiles io,
                                                         const DeviceTables *__restrict__ tab)
 {
   constexpr int P = 32;
   __shared__ __align__(16) int16_t s_poly[WAVES][P * kStride];
   __shared__ int16_t *s_ptr[WAVES][2 * P];
   __shared__ __align__(16) int32_t s_bias[16][16];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, h = lane >> 5, t = lane & 31;
+  const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), h = lane >> 5, t = lane & 31;
 
   const int32_t(*bias_c)[16] = INV ? tab->bias_inv2 : tab->bias_fwd2;
   for (int i = threadIdx.x; i < 256; i += 64 * WAVES)
@@ -312,7 +334,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_mfma32(Tiles io,
     }
     wave_sync();
     stage_load<P>(io, tile + stride, sr, lane);
-    tile_out<P>(L, [&](int pp) { return tile_dst<P>(io, tile, pp, s_ptr[w]); }, lane);
+    stage_out<P>(io, tile, L, s_ptr[w], lane);
     wave_sync();
   }
 }

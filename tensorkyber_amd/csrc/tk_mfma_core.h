@@ -71,6 +71,64 @@ __device__ __forceinline__ void tile_out(const int16_t *L, Addr dst, int lane)
   }
 }
 
+// Tile staging through a buffer descriptor over the tile's own bytes (P
+// consecutive polys, fewer in the last tile of a batch).  Accesses past the
+// descriptor's size read zero (loads) or are dropped (stores), so a partial
+// tile needs no per-lane checks: the loads and the stores go out as one
+// unconditional batch each.
+constexpr int kBufferRsrcWord3 = 0x00020000;  // gfx9 buffer descriptor: 32-bit data format
+constexpr int kBufferNT = 2;                  // cache policy: nontemporal
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t span_rsrc(const void *base, uint32_t bytes)
+{
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), 0, (int)bytes,
+                                           kBufferRsrcWord3);
+}
+
+// Groups of eight 16-byte accesses per lane (one 16-poly tile, or half of a
+// 32-poly one), with a scheduling barrier between groups so that at most one
+// group's registers are live.
+template <int P>
+__device__ __forceinline__ void tile_in_rsrc(int16_t *L, __amdgpu_buffer_rsrc_t r, int lane)
+{
+#pragma unroll
+  for (int g0 = 0; g0 < P / 2; g0 += 8) {
+    v4i v[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++)
+      v[i] = __builtin_bit_cast(v4i, __builtin_amdgcn_raw_buffer_load_b128(
+                                         r, lane * 16, (g0 + i) * 1024, kBufferNT));
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      const int q = (g0 + i) * 64 + lane, pp = q >> 5, ch = q & 31;
+      *reinterpret_cast<v4i *>(L + pp * kStride + ch * 8) = v[i];
+    }
+    if (g0 + 8 < P / 2)
+      __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+template <int P>
+__device__ __forceinline__ void tile_out_rsrc(const int16_t *L, __amdgpu_buffer_rsrc_t r, int lane)
+{
+  typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+#pragma unroll
+  for (int g0 = 0; g0 < P / 2; g0 += 8) {
+    v4i v[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      const int q = (g0 + i) * 64 + lane, pp = q >> 5, ch = q & 31;
+      v[i] = *reinterpret_cast<const v4i *>(L + pp * kStride + ch * 8);
+    }
+#pragma unroll
+    for (int i = 0; i < 8; i++)
+      __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4u, v[i]), r, lane * 16,
+                                             (g0 + i) * 1024, kBufferNT);
+    if (g0 + 8 < P / 2)
+      __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Packed small-coefficient input, the format of the reference's pack_small
 // (ntt_bench.cu:749-757): BITS-bit two's-complement fields, 8/BITS fields per

@@ -49,6 +49,13 @@ def test_ntt_packed_vs_oracle(tk, gpu, oracle, bits, method, npoly):
     assert np.array_equal(got, want), f"bits {bits} {method} npoly {npoly}"
     got_c = tk.ntt_packed(packed, bits, method=method, form="centered").cpu().numpy()
     assert np.array_equal(got_c, centred(want))
+    # partial last tile: nothing written past the batch
+    import torch
+    buf = torch.full((npoly + 8, N), 1234, dtype=torch.int16, device=gpu)
+    tk.ntt_packed(packed, bits, out=buf[:npoly], method=method)
+    host = buf.cpu().numpy()
+    assert np.array_equal(host[:npoly], want)
+    assert (host[npoly:] == 1234).all(), "kernel wrote past the batch"
 
 
 def test_ntt_packed_extremes(tk, gpu, oracle):

@@ -69,8 +69,13 @@ def test_ragged_tail_sizes(tk, gpu, oracle, method, npoly):
     view = buf[: npoly * N] if buf.dim() == 1 else buf[:npoly]
     tk.ntt(view, method=method)
     got = _host(buf)
-    assert np.array_equal(got[:npoly], canon(oracle.ntt(x)))
+    fwd = canon(oracle.ntt(x))
+    assert np.array_equal(got[:npoly], fwd)
     assert np.array_equal(got[npoly:], guard), "kernel wrote past the batch"
+    tk.invntt(view, method=method)
+    got = _host(buf)
+    assert np.array_equal(got[:npoly], centred(oracle.invntt(fwd)))
+    assert np.array_equal(got[npoly:], guard), "inverse wrote past the batch"
 
 
 @pytest.mark.parametrize("kind", ["uniform", "edge", "full"])
