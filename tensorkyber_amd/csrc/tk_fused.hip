@@ -159,6 +159,113 @@ __device__ __forceinline__ size_t role_stride(int role)
   }
 }
 
+__device__ __forceinline__ const int16_t *role_base(const FusedIo &io, int role)
+{
+  return role == rOut0   ? io.out0
+         : role == rOut1 ? io.out1
+         : role == rMat  ? io.mat
+         : role == rMat2 ? io.mat2
+         : role == rX    ? static_cast<const int16_t *>(io.x)
+         : role == rAdd0 ? io.add0
+         : role == rAdd1 ? io.add1
+                         : io.add2;
+}
+
+// The polys of one buffer role for the 16 instances of a tile, in the
+// staging lane order: chunk `it` of a tile covers instances 2*it and
+// 2*it + 1 (lanes 0-31 and 32-63), 16 bytes per lane.  Poly `sub` of an
+// instance is at + sub * 256 int16 from the instance's start.
+//
+// Contiguous layout (RoleTile<false>): instance i of the role starts at
+// base + i * stride.  A buffer descriptor spans the tile's valid instances,
+// so accesses of instances past the batch read zero or are dropped, with no
+// per-lane branch; a role with no buffer gets an empty descriptor (reads
+// zero).
+// Table layout (RoleTile<true>): the instances' pointers parked in LDS;
+// null pointers (instances past the batch, absent buffers) read zero and
+// are not written.
+template <bool TABLE>
+struct RoleTile;
+
+template <>
+struct RoleTile<false> {
+  __amdgpu_buffer_rsrc_t r;
+  int voff;  // lane's byte offset within chunk 0
+  int sb;    // per-instance stride in bytes
+  __device__ __forceinline__ RoleTile(const int16_t *base, size_t i0, int nvalid, int stride,
+                                      int lane)
+      : r(span_rsrc(base ? base + i0 * (size_t)stride : nullptr,
+                    base ? (uint32_t)nvalid * stride * 2 : 0u)),
+        voff((lane >> 5) * stride * 2 + (lane & 31) * 16),
+        sb(stride * 2)
+  {
+  }
+  __device__ __forceinline__ v4i load(int it, int sub) const
+  {
+    return __builtin_bit_cast(
+        v4i, __builtin_amdgcn_raw_buffer_load_b128(r, voff, 2 * it * sb + sub * 512, kBufferNT));
+  }
+  // The whole offset in the VGPR operand (see span_rsrc's note on stores).
+  __device__ __forceinline__ void store(int it, int sub, v4i v) const
+  {
+    typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4u, v), r,
+                                           voff + 2 * it * sb + sub * 512, 0, kBufferNT);
+  }
+};
+
+template <>
+struct RoleTile<true> {
+  const int16_t *const (*ptrs)[8];  // [instance][role], in LDS
+  int role, lane;
+  __device__ __forceinline__ const int16_t *at(int it, int sub) const
+  {
+    const int16_t *p = ptrs[2 * it + (lane >> 5)][role];
+    return p ? p + sub * kN + (lane & 31) * 8 : nullptr;
+  }
+  __device__ __forceinline__ v4i load(int it, int sub) const
+  {
+    const int16_t *p = at(it, sub);
+    return p ? ld_stream(reinterpret_cast<const v4i *>(p)) : v4i{0, 0, 0, 0};
+  }
+  __device__ __forceinline__ void store(int it, int sub, v4i v) const
+  {
+    int16_t *p = const_cast<int16_t *>(at(it, sub));
+    if (p)
+      st_stream(reinterpret_cast<v4i *>(p), v);
+  }
+};
+
+// Poly `sub` of the tile's instances into / out of the slab.  All loads (or
+// LDS reads) are issued before the first use.
+template <int P, class RT>
+__device__ __forceinline__ void role_in(int16_t *slab, const RT &rt, int sub, int lane)
+{
+  v4i v[P / 2];
+#pragma unroll
+  for (int it = 0; it < P / 2; it++)
+    v[it] = rt.load(it, sub);
+#pragma unroll
+  for (int it = 0; it < P / 2; it++) {
+    const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
+    *reinterpret_cast<v4i *>(slab + pp * kStride + ch * 8) = v[it];
+  }
+}
+
+template <int P, class RT>
+__device__ __forceinline__ void role_out(const int16_t *slab, const RT &rt, int sub, int lane)
+{
+  v4i v[P / 2];
+#pragma unroll
+  for (int it = 0; it < P / 2; it++) {
+    const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
+    v[it] = *reinterpret_cast<const v4i *>(slab + pp * kStride + ch * 8);
+  }
+#pragma unroll
+  for (int it = 0; it < P / 2; it++)
+    rt.store(it, sub, v[it]);
+}
+
 // K: module rank; MODE: kFusedMatvec / Keygen / Encrypt / Decrypt (FusedIo
 // documents the buffers of each); SBITS = 0: the transformed vector holds
 // int16 polys [ninst][K][256]; 2, 3, 4: the packed small-coefficient form
@@ -215,34 +322,19 @@ __global__ __launch_bounds__((64 * fused_waves<MODE, K>())) void k_fused(
 
   int16_t *const slab = s_slab[w];
   int16_t *const pslab = slab + tt * kStride;
-  // Buffer of a role for tile instance pp (i0 and the parity of the tile
-  // are the loop's): the table's pointer, or base + instance * stride.
+  // Accessor of a role for the current tile (i0, nvalid and the parity of
+  // the tile are the loop's).
   size_t i0 = 0;
-  int par = 0;
-  auto ip = [&](int role, int pp) -> int16_t * {
-    if constexpr (TABLE) {
-      return const_cast<int16_t *>(s_ip[par][pp][role]);
-    } else {
-      const int16_t *base = role == rOut0   ? io.out0
-                            : role == rOut1 ? io.out1
-                            : role == rMat  ? io.mat
-                            : role == rMat2 ? io.mat2
-                            : role == rX    ? static_cast<const int16_t *>(io.x)
-                            : role == rAdd0 ? io.add0
-                            : role == rAdd1 ? io.add1
-                                            : io.add2;
-      return base ? const_cast<int16_t *>(base) + (i0 + pp) * role_stride<MODE, K>(role) : nullptr;
-    }
-  };
-  // Matrix row of output row w for tile instance pp (K consecutive polys).
-  auto row_base = [&](int pp) -> const int16_t * {
-    if constexpr (MODE == kFusedEncrypt)
-      return w < K ? ip(rMat, pp) + (size_t)w * K * kN : ip(rMat2, pp);
-    else if constexpr (MODE == kFusedDecrypt)
-      return ip(rMat, pp);
+  int par = 0, nvalid = 0;
+  auto role = [&](int rl) {
+    if constexpr (TABLE)
+      return RoleTile<true>{s_ip[par], rl, lane};
     else
-      return ip(rMat, pp) + (size_t)w * K * kN;
+      return RoleTile<false>(role_base(io, rl), i0, nvalid, (int)role_stride<MODE, K>(rl), lane);
   };
+  // Matrix row of output row w: the role and the first of its K polys.
+  const int row_role = (MODE == kFusedEncrypt && w == K) ? rMat2 : rMat;
+  const int row_sub = (MODE == kFusedEncrypt && w == K) ? 0 : w * K;
   // Forward transform of the staged slab to canonical values.
   auto forward = [&]() {
     wave_sync();
@@ -255,7 +347,7 @@ __global__ __launch_bounds__((64 * fused_waves<MODE, K>())) void k_fused(
   const size_t ntiles = (io.ninst + P - 1) / P;
   for (size_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x, par ^= (TABLE ? 1 : 0)) {
     i0 = tile * P;
-    const int nvalid = (int)(io.ninst - i0 < (size_t)P ? io.ninst - i0 : P);
+    nvalid = (int)(io.ninst - i0 < (size_t)P ? io.ninst - i0 : P);
     if constexpr (TABLE) {
       // Buffer par was last read in the tile before the previous one, whose
       // readers have all passed the previous tile's barriers.
@@ -272,22 +364,26 @@ __global__ __launch_bounds__((64 * fused_waves<MODE, K>())) void k_fused(
     // 1. x_hat[w] = ntt(x_w), canonical, and its gamma products.
     if (w < K) {
       if constexpr (SBITS == 0) {
-        tile_in<P>(slab,
-                   [&](int pp) -> const int16_t * {
-                     return pp < nvalid ? ip(rX, pp) + (size_t)w * kN : nullptr;
-                   },
-                   lane);
+        role_in<P>(slab, role(rX), w, lane);
         forward();
       } else {
-        constexpr size_t PPB = packed_poly_bytes<SBITS>();
-        uint32_t pv[packed_lane_dwords<SBITS>()];
-        packed_load_lanes<SBITS>(pv,
-                                 [&](int pp) -> const uint8_t * {
-                                   return pp < nvalid ? static_cast<const uint8_t *>(io.x) +
-                                                            ((i0 + pp) * K + w) * PPB
-                                                      : nullptr;
-                                 },
-                                 lane);
+        // Lane (g, t) takes the packed bytes of rows 4g..4g+3 of poly w of
+        // instance t (packed_load_lanes' layout), through a descriptor over
+        // the tile's valid instances.
+        constexpr int PPB = packed_poly_bytes<SBITS>(), NDW = packed_lane_dwords<SBITS>();
+        const __amdgpu_buffer_rsrc_t xr =
+            span_rsrc(static_cast<const uint8_t *>(io.x) + i0 * K * PPB, (uint32_t)nvalid * K * PPB);
+        uint32_t pv[NDW];
+#pragma unroll
+        for (int i = 0; i < NDW / 4; i++) {
+          const v4i x = __builtin_bit_cast(
+              v4i, __builtin_amdgcn_raw_buffer_load_b128(xr, tt * K * PPB + g * NDW * 4,
+                                                         w * PPB + i * 16, kBufferNT));
+          pv[4 * i] = (uint32_t)x[0];
+          pv[4 * i + 1] = (uint32_t)x[1];
+          pv[4 * i + 2] = (uint32_t)x[2];
+          pv[4 * i + 3] = (uint32_t)x[3];
+        }
         pass_m16_packed<SBITS>(pslab, g, pv, f1pklo, f1pkhi, fb1pk);
         wave_sync();
         pass_c16<kFormCanon>(pslab, g, f2lo, f2hi, s_bf2);
@@ -300,13 +396,11 @@ __global__ __launch_bounds__((64 * fused_waves<MODE, K>())) void k_fused(
       }
       if constexpr (MODE == kFusedKeygen) {
         // s_hat = poly_ntt(s_w): the centred representatives.
+        const auto o1 = role(rOut1);
 #pragma unroll
         for (int it = 0; it < P / 2; it++) {
           const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
-          const v4i v = *reinterpret_cast<const v4i *>(slab + pp * kStride + ch * 8);
-          if (pp < nvalid)
-            st_stream(reinterpret_cast<v4i *>(ip(rOut1, pp) + (size_t)w * kN) + ch,
-                      barrett_v4(v));
+          o1.store(it, w, barrett_v4(*reinterpret_cast<const v4i *>(slab + pp * kStride + ch * 8)));
         }
       }
     }
@@ -321,11 +415,8 @@ __global__ __launch_bounds__((64 * fused_waves<MODE, K>())) void k_fused(
       // partial s_hat_w o u_hat_w against its own slab and then overwrites
       // that slab with it (no other wave reads it); wave 0 sums the partials
       // after the barrier.
-      auto load_part = [&](v4i &a, int it) {
-        const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
-        a = pp < nvalid ? ld_stream(reinterpret_cast<const v4i *>(ip(rMat, pp) + (size_t)w * kN) + ch)
-                        : v4i{0, 0, 0, 0};
-      };
+      const auto um = role(rMat);
+      auto load_part = [&](v4i &a, int it) { a = um.load(it, w); };
       v4i a_cur, a_nxt;
       load_part(a_cur, 0);
 #pragma unroll
@@ -354,12 +445,11 @@ __global__ __launch_bounds__((64 * fused_waves<MODE, K>())) void k_fused(
             v4i{(int)res[it][0], (int)res[it][1], (int)res[it][2], (int)res[it][3]};
       }
     } else if (w < R) {
+      const auto mr = role(row_role);
       auto load_row = [&](v4i *a, int it) {
-        const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
-        const v4i *arow = reinterpret_cast<const v4i *>(row_base(pp)) + ch;
 #pragma unroll
         for (int j = 0; j < K; j++)
-          a[j] = pp < nvalid ? ld_stream(arow + j * (kN / 8)) : v4i{0, 0, 0, 0};
+          a[j] = mr.load(it, row_sub + j);
       };
       v4i a_cur[K], a_nxt[K];
       load_row(a_cur, 0);
@@ -418,12 +508,9 @@ __global__ __launch_bounds__((64 * fused_waves<MODE, K>())) void k_fused(
     if (w < R) {
       if constexpr (MODE == kFusedKeygen) {
         // t_hat_w = reduce(row + poly_ntt(e_w)), NTT domain.
-        tile_in<P>(slab,
-                   [&](int pp) -> const int16_t * {
-                     return pp < nvalid ? ip(rAdd0, pp) + (size_t)w * kN : nullptr;
-                   },
-                   lane);
+        role_in<P>(slab, role(rAdd0), w, lane);
         forward();
+        const auto o0 = role(rOut0);
 #pragma unroll
         for (int it = 0; it < P / 2; it++) {
           const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
@@ -431,9 +518,7 @@ __global__ __launch_bounds__((64 * fused_waves<MODE, K>())) void k_fused(
           const v4i sum = add16(v4i{(int)res[it][0], (int)res[it][1], (int)res[it][2],
                                     (int)res[it][3]},
                                 e);  // < 3q: no wrap
-          if (pp < nvalid)
-            st_stream(reinterpret_cast<v4i *>(ip(rOut0, pp) + (size_t)w * kN) + ch,
-                      barrett_v4(sum));
+          o0.store(it, w, barrett_v4(sum));
         }
       } else {
 #pragma unroll
@@ -448,37 +533,30 @@ __global__ __launch_bounds__((64 * fused_waves<MODE, K>())) void k_fused(
         pass_m16<kFormCentred>(pslab, g, i1lo, i1hi, ib1);
         wave_sync();
         if constexpr (MODE == kFusedMatvec) {
-          tile_out<P>(slab,
-                      [&](int pp) -> int16_t * {
-                        return pp < nvalid ? ip(rOut0, pp) + (size_t)w * kN : nullptr;
-                      },
-                      lane);
+          role_out<P>(slab, role(rOut0), w, lane);
         } else {
+          // Addends of the output row: e1_w (a u row), e2 and m (the v row),
+          // v (decrypt).  All loads go out before the first sum.
+          const bool vrow = MODE == kFusedEncrypt && w == K;
+          const int sub = (MODE == kFusedEncrypt && !vrow) ? w : 0;
+          const auto ya = role(vrow ? rAdd1 : rAdd0);
+          const auto za = role(rAdd2);
+          const auto oa = role(vrow ? rOut1 : rOut0);
+          const v4i zero = {0, 0, 0, 0};
+          v4i y[P / 2], z[P / 2];
+#pragma unroll
+          for (int it = 0; it < P / 2; it++) {
+            y[it] = ya.load(it, sub);
+            z[it] = vrow ? za.load(it, 0) : zero;
+          }
 #pragma unroll
           for (int it = 0; it < P / 2; it++) {
             const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
-            if (pp >= nvalid)
-              continue;
             const v4i x = *reinterpret_cast<const v4i *>(slab + pp * kStride + ch * 8);
-            const v4i zero = {0, 0, 0, 0};
-            if constexpr (MODE == kFusedEncrypt) {
-              if (w < K) {  // u_w = reduce(x + e1_w)
-                const v4i e1 =
-                    ld_stream(reinterpret_cast<const v4i *>(ip(rAdd0, pp) + (size_t)w * kN) + ch);
-                st_stream(reinterpret_cast<v4i *>(ip(rOut0, pp) + (size_t)w * kN) + ch,
-                          centred_sum<1>(x, e1, zero));
-              } else {  // v = reduce(x + e2 + m)
-                const v4i e2 = ld_stream(reinterpret_cast<const v4i *>(ip(rAdd1, pp)) + ch);
-                const int16_t *mp = ip(rAdd2, pp);
-                const v4i m = mp ? ld_stream(reinterpret_cast<const v4i *>(mp) + ch) : zero;
-                st_stream(reinterpret_cast<v4i *>(ip(rOut1, pp)) + ch,
-                          centred_sum<1>(x, e2, m));
-              }
-            } else {  // decrypt: w = reduce(v - x)
-              const v4i vin = ld_stream(reinterpret_cast<const v4i *>(ip(rAdd0, pp)) + ch);
-              st_stream(reinterpret_cast<v4i *>(ip(rOut0, pp)) + ch,
-                        centred_sum<-1>(x, vin, zero));
-            }
+            if constexpr (MODE == kFusedEncrypt)  // u_w = reduce(x + e1_w), v = reduce(x + e2 + m)
+              oa.store(it, sub, centred_sum<1>(x, y[it], z[it]));
+            else  // decrypt: w = reduce(v - x)
+              oa.store(it, sub, centred_sum<-1>(x, y[it], zero));
           }
         }
       }
