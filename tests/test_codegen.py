@@ -1,0 +1,120 @@
+"""Code-generation checks on the gfx950 kernels (CPU only: hipcc cross-compiles).
+
+Each HIP source is compiled to assembly with the library's own flags and the
+compiler's resource remarks.  Two properties that GPU tests only catch by
+luck are checked on every kernel:
+
+* no spills to memory, and the tile kernels at their occupancy targets (three
+  waves per SIMD for the MFMA16 kernels): a spill or a lost wave costs 5-15%
+  on the streaming kernels without changing any result;
+* no store of more than 8 bytes whose data VGPRs the very next VALU
+  instruction overwrites.  On gfx950 such a store writes the new values; the
+  compiler only guards stores without a scalar offset register, so a buffer
+  store with one can lose its data under load (seen as whole 16-byte chunks of
+  garbage in a full-size run, never in small ones).
+"""
+import concurrent.futures as cf
+import os
+import re
+import shutil
+import subprocess
+
+import pytest
+
+from tensorkyber_amd import build as tkbuild
+
+HIPCC = tkbuild.HIPCC if os.path.exists(tkbuild.HIPCC) else shutil.which("hipcc")
+pytestmark = pytest.mark.skipif(HIPCC is None, reason="hipcc not available")
+
+KERNEL_SOURCES = sorted(f for f in os.listdir(tkbuild.CSRC) if f.endswith(".hip"))
+
+
+def _compile(src, tmp):
+    asm = os.path.join(tmp, src + ".s")
+    flags = [f for f in tkbuild.HIP_FLAGS if f != "-fPIC"]
+    cmd = [HIPCC, *flags, "--cuda-device-only", "-S", "-Rpass-analysis=kernel-resource-usage",
+           os.path.join(tkbuild.CSRC, src), "-o", asm]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-2000:]
+    with open(asm) as f:
+        return src, f.read(), r.stderr
+
+
+@pytest.fixture(scope="module")
+def compiled(tmp_path_factory):
+    tmp = str(tmp_path_factory.mktemp("asm"))
+    with cf.ThreadPoolExecutor(min(8, len(KERNEL_SOURCES))) as ex:
+        return list(ex.map(lambda s: _compile(s, tmp), KERNEL_SOURCES))
+
+
+def _resources(remarks):
+    """{kernel: {field: value}} from the kernel-resource-usage remarks."""
+    out, name = {}, None
+    for line in remarks.splitlines():
+        m = re.search(r"Function Name: (\S+)", line)
+        if m:
+            name = m.group(1)
+            out[name] = {}
+            continue
+        m = re.search(r"remark:\s+([A-Za-z \[\]/]+): (\d+)", line)
+        if m and name:
+            out[name][m.group(1).strip()] = int(m.group(2))
+    return out
+
+
+def test_no_spills_and_tile_kernel_occupancy(compiled):
+    seen = 0
+    for src, _, remarks in compiled:
+        for name, r in _resources(remarks).items():
+            seen += 1
+            # SGPR spills go to VGPR lanes; VGPR spills and any scratch go to memory
+            assert r.get("VGPRs Spill", 0) == 0, f"{src}: {name} spills VGPRs"
+            assert r.get("ScratchSize [bytes/lane]", 0) == 0, f"{src}: {name} uses scratch"
+            if "k_mfma16" in name:
+                assert r["Occupancy [waves/SIMD]"] >= 3, f"{name}: occupancy below 3"
+            if "k_mfma32" in name:
+                assert r["Occupancy [waves/SIMD]"] >= 2, f"{name}: occupancy below 2"
+    assert seen > 50, "kernel remarks not found"
+
+
+_WIDE_STORE = re.compile(r"^(buffer|global|flat)_store_dwordx(3|4)\b.*?v\[(\d+):(\d+)\]")
+_VALU_DST = re.compile(r"^v_\S+\s+v\[?(\d+)(?::(\d+))?")
+
+
+def _wide_store_hazards(asm):
+    found, stores = [], 0
+    for m in re.finditer(r"^(_Z\w+):", asm, re.M):
+        body = asm[m.end():asm.index(".Lfunc_end", m.end())]
+        ins = [l.strip() for l in body.split("\n")
+               if l.startswith("\t") and not l.strip().startswith((".", ";"))]
+        for j, line in enumerate(ins[:-1]):
+            st = _WIDE_STORE.match(line)
+            if not st:
+                continue
+            stores += 1
+            lo, hi = int(st.group(3)), int(st.group(4))
+            d = _VALU_DST.match(ins[j + 1])
+            if d:
+                a, b = int(d.group(1)), int(d.group(2) or d.group(1))
+                if a <= hi and b >= lo:
+                    found.append(f"{m.group(1)}: {line} || {ins[j + 1]}")
+    return found, stores
+
+
+def test_wide_store_data_not_overwritten_next(compiled):
+    total = 0
+    for src, asm, _ in compiled:
+        found, stores = _wide_store_hazards(asm)
+        total += stores
+        assert not found, f"{src}:\n" + "\n".join(found[:5])
+    assert total > 100, "no wide stores found: the scan is not reading the assembly"
+
+
+def test_scan_detects_the_pattern():
+    """The scan itself flags a store followed at once by a write of its data."""
+    asm = ("_Zfake:\n\tbuffer_store_dwordx4 v[8:11], v43, s[20:23], s65 offen nt\n"
+           "\tv_bfe_i32 v9, v4, 0, 16\n\ts_endpgm\n.Lfunc_end0:\n")
+    found, stores = _wide_store_hazards(asm)
+    assert stores == 1 and len(found) == 1
+    safe = asm.replace("\tv_bfe_i32 v9", "\ts_nop 0\n\tv_bfe_i32 v9")
+    assert _wide_store_hazards(safe)[0] == []
