@@ -310,6 +310,7 @@ __device__ __forceinline__ void pass_m16(int16_t *P, int g, long alo, long ahi, 
               finish2<MODE>(pack2(v[0], v[G - 1]));
       }
     }
+    __builtin_amdgcn_sched_barrier(0);  // one column quad's operands live at a time
   }
 }
 
