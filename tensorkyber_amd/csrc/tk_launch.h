@@ -19,7 +19,11 @@ struct LinearTiles {
 // A ragged batch of requests, transformed in place: the requests' polys are
 // numbered in table order, and ptrs[g] is the device address of poly g.  One
 // table read per poly (8 of every 1,032 bytes moved) replaces a dependent
-// walk of the request table in front of every tile.
+// walk of the request table in front of every tile.  The table holds npoly
+// rounded up to a multiple of kRaggedPad entries; those past npoly point at
+// a scratch poly of the plan, so the last tile loads and stores like the
+// others instead of branching per poly.
+constexpr int kRaggedPad = 32;  // the largest tile, in polys
 struct RaggedTiles {
   int16_t *const *ptrs;
   size_t npoly;

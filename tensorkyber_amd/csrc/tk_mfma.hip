@@ -58,8 +58,9 @@ __device__ __forceinline__ void tile_prepare(const LinearTiles &, size_t, int, i
 template <int P>
 __device__ __forceinline__ int16_t *tile_fetch(const RaggedTiles &io, size_t tile, int lane)
 {
+  static_assert(kRaggedPad % P == 0, "the padded table covers every tile");
   const uint64_t g = (uint64_t)tile * P + lane;
-  return lane < P && g < io.npoly ? io.ptrs[g] : nullptr;
+  return lane < P && (uint64_t)tile * P < io.npoly ? io.ptrs[g] : nullptr;
 }
 
 template <int P>
@@ -157,12 +158,24 @@ __device__ __forceinline__ void stage_in(const LinearTiles &io, size_t tile, int
   tile_in_rsrc<P>(L, span_rsrc(io.in + tile * P * kN, tile_bytes<P>(tile, io.npoly, 2 * kN)), lane);
 }
 
+// Ragged tiles: every parked address is valid (padding points at the
+// plan's scratch poly), so the loads and stores are unconditional.
 template <int P>
 __device__ __forceinline__ void stage_in(const RaggedTiles &io, size_t tile, int16_t *L,
                                          int16_t *const *lds, const StageRegs<P, RaggedTiles> &,
                                          int lane)
 {
-  tile_in<P>(L, [&](int pp) { return tile_src<P>(io, tile, pp, lds); }, lane);
+  v4i v[P / 2];
+#pragma unroll
+  for (int it = 0; it < P / 2; it++) {
+    const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
+    v[it] = ld_stream(reinterpret_cast<const v4i *>(tile_src<P>(io, tile, pp, lds)) + ch);
+  }
+#pragma unroll
+  for (int it = 0; it < P / 2; it++) {
+    const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
+    *reinterpret_cast<v4i *>(L + pp * kStride + ch * 8) = v[it];
+  }
 }
 
 template <int P, int BITS>
@@ -195,7 +208,17 @@ template <int P>
 __device__ __forceinline__ void stage_out(const RaggedTiles &io, size_t tile, const int16_t *L,
                                           int16_t *const *lds, int lane)
 {
-  tile_out<P>(L, [&](int pp) { return tile_dst<P>(io, tile, pp, lds); }, lane);
+  v4i v[P / 2];
+#pragma unroll
+  for (int it = 0; it < P / 2; it++) {
+    const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
+    v[it] = *reinterpret_cast<const v4i *>(L + pp * kStride + ch * 8);
+  }
+#pragma unroll
+  for (int it = 0; it < P / 2; it++) {
+    const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
+    st_stream(reinterpret_cast<v4i *>(tile_dst<P>(io, tile, pp, lds)) + ch, v[it]);
+  }
 }
 
 // Bits per coefficient of packed tiles, 0 for int16 tiles.

@@ -15,62 +15,10 @@ constexpr int kStride = 264;  // int16 per poly slot in LDS (512 + 16 bytes)
 constexpr int kPassGroup = TK_PASS_GROUP;  // MFMA pairs in flight per pass step
 
 // ---------------------------------------------------------------------------
-// Tile staging: P polys, coalesced 16-byte global accesses, one LDS slab.
-// `src` yields the address of tile poly pp (nullptr = beyond the batch).
+// Tile staging: P polys, coalesced 16-byte global accesses, one LDS slab
+// (poly pp at pp * kStride).  Chunk `it` of a tile is polys 2*it and 2*it+1,
+// lanes 0-31 and 32-63, 16 bytes per lane.
 // ---------------------------------------------------------------------------
-template <int P, class Addr>
-__device__ __forceinline__ void tile_in(int16_t *L, Addr src, int lane)
-{
-  v4i v[P / 2];
-#pragma unroll
-  for (int it = 0; it < P / 2; it++) {
-    const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
-    const int16_t *p = src(pp);
-    v[it] = p ? ld_stream(reinterpret_cast<const v4i *>(p) + ch) : v4i{0, 0, 0, 0};
-  }
-#pragma unroll
-  for (int it = 0; it < P / 2; it++) {
-    const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
-    *reinterpret_cast<v4i *>(L + pp * kStride + ch * 8) = v[it];
-  }
-}
-
-// Split form of tile_in for software pipelining: tile_load issues the global
-// loads of a tile into registers, tile_park writes them to the LDS slab.
-template <int P, class Addr>
-__device__ __forceinline__ void tile_load(v4i *v, Addr src, int lane)
-{
-#pragma unroll
-  for (int it = 0; it < P / 2; it++) {
-    const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
-    const int16_t *p = src(pp);
-    v[it] = p ? ld_stream(reinterpret_cast<const v4i *>(p) + ch) : v4i{0, 0, 0, 0};
-  }
-}
-
-template <int P>
-__device__ __forceinline__ void tile_park(int16_t *L, const v4i *v, int lane)
-{
-#pragma unroll
-  for (int it = 0; it < P / 2; it++) {
-    const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
-    *reinterpret_cast<v4i *>(L + pp * kStride + ch * 8) = v[it];
-  }
-}
-
-template <int P, class Addr>
-__device__ __forceinline__ void tile_out(const int16_t *L, Addr dst, int lane)
-{
-#pragma unroll
-  for (int it = 0; it < P / 2; it++) {
-    const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
-    const v4i v = *reinterpret_cast<const v4i *>(L + pp * kStride + ch * 8);
-    int16_t *p = dst(pp);
-    if (p)
-      st_stream(reinterpret_cast<v4i *>(p) + ch, v);
-  }
-}
-
 // Tile staging through a buffer descriptor over the tile's own bytes (P
 // consecutive polys, fewer in the last tile of a batch).  Accesses past the
 // descriptor's size read zero (loads) or are dropped (stores), so a partial
@@ -205,14 +153,6 @@ __device__ __forceinline__ void packed_park(int16_t *L, const uint32_t *v, int l
     const int q = it * 64 + lane, pp = q / DPP, wd = q % DPP;
     unpack_dword<BITS>(v[it], L + pp * kStride + wd * CPD);
   }
-}
-
-template <int P, int BITS, class Addr>
-__device__ __forceinline__ void tile_in_packed(int16_t *L, Addr src, int lane)
-{
-  uint32_t v[packed_regs<P, BITS>()];
-  packed_load<P, BITS>(v, src, lane);
-  packed_park<P, BITS>(L, v, lane);
 }
 
 typedef unsigned short us2 __attribute__((ext_vector_type(2)));

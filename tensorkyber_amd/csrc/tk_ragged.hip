@@ -235,10 +235,15 @@ int tk_ragged_plan_create(tk_ragged_plan **plan, const tk_request *table, size_t
   }
   const int shape = tv.method == TK_METHOD_MFMA32 ? 32 : 16;
 
-  // Host image of the address table: poly g of the flattened numbering.
+  // Host image of the address table: poly g of the flattened numbering,
+  // padded to a multiple of kRaggedPad entries that point at the plan's
+  // scratch poly (after the table in the same allocation).
+  const size_t padded = (total + kRaggedPad - 1) / kRaggedPad * kRaggedPad;
+  const size_t b_ptrs = padded * sizeof(int16_t *);
+  const size_t b_alloc = b_ptrs + kN * sizeof(int16_t);
   std::vector<int16_t *> ptrs;
   try {
-    ptrs.resize(total);
+    ptrs.resize(padded);
   } catch (const std::bad_alloc &) {
     return TK_ENOMEM;
   }
@@ -246,7 +251,6 @@ int tk_ragged_plan_create(tk_ragged_plan **plan, const tk_request *table, size_t
   for (size_t r = 0; r < nreq; r++)
     for (uint32_t i = 0; i < table[r].npoly; i++)
       ptrs[g++] = table[r].base + (size_t)i * kN;
-  const size_t b_ptrs = total * sizeof(int16_t *);
 
   tk_ragged_plan *p = new tk_ragged_plan();
   p->device = st->device;
@@ -258,10 +262,13 @@ int tk_ragged_plan_create(tk_ragged_plan **plan, const tk_request *table, size_t
   p->npoly = total;
   p->nreq = nreq;
   p->tab = st->tables;
-  if (hipMalloc(&p->dmem, b_ptrs) != hipSuccess) {
+  if (hipMalloc(&p->dmem, b_alloc) != hipSuccess) {
     plan_free(p);
     return TK_ENOMEM;
   }
+  int16_t *scratch = reinterpret_cast<int16_t *>(static_cast<char *>(p->dmem) + b_ptrs);
+  for (size_t i = total; i < padded; i++)
+    ptrs[i] = scratch;
   if ((rc = record_hip(hipMemcpy(p->dmem, ptrs.data(), b_ptrs, hipMemcpyHostToDevice)))) {
     plan_free(p);
     return rc;
