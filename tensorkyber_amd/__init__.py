@@ -228,6 +228,8 @@ def matvec(A_hat, s, k: int, out=None, stream=None):
         raise ValueError("A_hat and s shapes disagree")
     torch = _torch()
     out = torch.empty_like(s) if out is None else _as_polys(out, "out")
+    if s.numel() != ninst * k * N or out.numel() != s.numel():
+        raise ValueError("s and out must hold ninst*k polys")
     _check(lib().matvec_ntt_fused(out.data_ptr(), A_hat.data_ptr(), s.data_ptr(), k, ninst,
                                   _stream_ptr(stream)), "matvec_ntt_fused")
     return out
@@ -291,6 +293,8 @@ def matvec_packed(A_hat, s_packed, k: int, bits: int, out=None, stream=None):
     torch = _torch()
     out = torch.empty((ninst * k, N), dtype=torch.int16, device=A_hat.device) if out is None \
         else _as_polys(out, "out")
+    if out.numel() != ninst * k * N:
+        raise ValueError("out must hold ninst*k polys")
     _check(lib().matvec_ntt_fused_packed(out.data_ptr(), A_hat.data_ptr(), s_packed.data_ptr(), k,
                                          ninst, bits, _stream_ptr(stream)),
            "matvec_ntt_fused_packed")
@@ -354,6 +358,8 @@ def kpke_keygen(A_hat, s, e, k: int, t_hat=None, s_hat=None, stream=None):
     torch = _torch()
     t_hat = torch.empty_like(s) if t_hat is None else _as_polys(t_hat, "t_hat")
     s_hat = torch.empty_like(s) if s_hat is None else _as_polys(s_hat, "s_hat")
+    if t_hat.numel() != s.numel() or s_hat.numel() != s.numel():
+        raise ValueError("t_hat and s_hat must hold ninst*k polys")
     _check(lib().kpke_keygen_core(t_hat.data_ptr(), s_hat.data_ptr(), A_hat.data_ptr(),
                                   s.data_ptr(), e.data_ptr(), k, ninst, _stream_ptr(stream)),
            "kpke_keygen_core")
@@ -374,6 +380,8 @@ def kpke_encrypt(At_hat, t_hat, r, e1, e2, m, k: int, u=None, v=None, stream=Non
         else _as_polys(u, "u")
     v = torch.empty((ninst, N), dtype=torch.int16, device=r.device) if v is None \
         else _as_polys(v, "v")
+    if u.numel() != ninst * k * N or v.numel() != ninst * N:
+        raise ValueError("u must hold ninst*k polys and v ninst polys")
     _check(lib().kpke_encrypt_core(u.data_ptr(), v.data_ptr(), At_hat.data_ptr(), t_hat.data_ptr(),
                                    r.data_ptr(), e1.data_ptr(), e2.data_ptr(), _ptr_or_none(m), k,
                                    ninst, _stream_ptr(stream)), "kpke_encrypt_core")
@@ -389,6 +397,8 @@ def kpke_decrypt(s_hat, u, v, k: int, w=None, stream=None):
     torch = _torch()
     w = torch.empty((ninst, N), dtype=torch.int16, device=u.device) if w is None \
         else _as_polys(w, "w")
+    if w.numel() != ninst * N:
+        raise ValueError("w must hold ninst polys")
     _check(lib().kpke_decrypt_core(w.data_ptr(), s_hat.data_ptr(), u.data_ptr(), v.data_ptr(), k,
                                    ninst, _stream_ptr(stream)), "kpke_decrypt_core")
     return w

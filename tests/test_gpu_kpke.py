@@ -117,3 +117,61 @@ def test_kpke_full_size_digests(tk, gpu, digests):
         x = t.cpu().numpy()
         assert np.array_equal(x, centred(x)), f"{name} not centred"
         assert tk.canon_digest(x) == d[name], name
+
+
+@pytest.mark.parametrize("k,ninst", [(2, 1), (3, 17), (4, 33)])
+def test_fused_outputs_stop_at_the_batch(tk, gpu, oracle, k, ninst):
+    """Partial last tiles: every output right and nothing written past any of
+    them (the kernels rely on the buffer range check for the lanes of
+    instances past the batch)."""
+    import torch
+    G = 8  # guard polys after each output
+
+    def guarded(npolys):
+        buf = torch.full((npolys + G, N), 1234, dtype=torch.int16, device=gpu)
+        return buf, buf[:npolys]
+
+    def untouched(buf, npolys, name):
+        assert (buf[npolys:] == 1234).all().item(), f"{name}: written past the batch"
+
+    w = tk.kpke_workload(ninst, k, seed=470 + k + ninst)
+    want_t, want_s = oracle.kpke_keygen(w["A_hat"], w["s"], w["e"], k)
+    tb, t_hat = guarded(ninst * k)
+    sb, s_hat = guarded(ninst * k)
+    tk.kpke_keygen(_dev(w["A_hat"], gpu), _dev(w["s"], gpu), _dev(w["e"], gpu), k,
+                   t_hat=t_hat, s_hat=s_hat)
+    assert np.array_equal(_np(t_hat, want_t.shape), want_t)
+    assert np.array_equal(_np(s_hat, want_s.shape), want_s)
+    untouched(tb, ninst * k, "t_hat")
+    untouched(sb, ninst * k, "s_hat")
+
+    At = _transpose(w["A_hat"])
+    want_u, want_v = oracle.kpke_encrypt(At, want_t, w["r"], w["e1"], w["e2"], w["m"], k)
+    ub, u = guarded(ninst * k)
+    vb, v = guarded(ninst)
+    tk.kpke_encrypt(_dev(At, gpu), t_hat, _dev(w["r"], gpu), _dev(w["e1"], gpu),
+                    _dev(w["e2"], gpu), _dev(w["m"], gpu), k, u=u, v=v)
+    assert np.array_equal(_np(u, want_u.shape), want_u)
+    assert np.array_equal(_np(v, want_v.shape), want_v)
+    untouched(ub, ninst * k, "u")
+    untouched(vb, ninst, "v")
+
+    want_w = oracle.kpke_decrypt(want_s, want_u, want_v, k)
+    wb, dec = guarded(ninst)
+    tk.kpke_decrypt(s_hat, u, v, k, w=dec)
+    assert np.array_equal(_np(dec, want_w.shape), want_w)
+    untouched(wb, ninst, "w")
+
+    # the matrix-vector product, int16 and packed secrets
+    A = w["A_hat"]
+    s = w["s"]
+    want_mv = centred(oracle.matvec(A, s, k)).reshape(ninst * k, N)
+    mb, mv = guarded(ninst * k)
+    tk.matvec(_dev(A, gpu), _dev(s, gpu), k, out=mv)
+    assert np.array_equal(mv.cpu().numpy(), want_mv)
+    untouched(mb, ninst * k, "matvec t")
+    pb, pmv = guarded(ninst * k)
+    packed = torch.from_numpy(oracle.pack_small(s.reshape(ninst * k, N), 3)).to(gpu)
+    tk.matvec_packed(_dev(A, gpu), packed, k, 3, out=pmv)
+    assert np.array_equal(pmv.cpu().numpy(), want_mv)
+    untouched(pb, ninst * k, "packed matvec t")
