@@ -338,6 +338,7 @@ __device__ __forceinline__ void pass_m16_packed(int16_t *P, int g, const uint32_
         val[u] = reduce_2q(lo[u][r], hi[u][r]);
       st2(rows + 16 * r + 4 * cq, pack2(val[0], val[1]), pack2(val[2], val[3]));
     }
+    __builtin_amdgcn_sched_barrier(0);  // one column quad's operands live at a time
   }
 }
 
