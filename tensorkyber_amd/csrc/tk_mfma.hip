@@ -229,8 +229,9 @@ template <int BITS>
 struct PackedBits<PackedTiles<BITS>> : std::integral_constant<int, BITS> {
 };
 
-// Three waves per SIMD (at most 168 VGPRs): without the bound the scheduler
-// trades a wave of occupancy for longer-lived staging registers.
+// At least three waves per SIMD (at most 168 VGPRs): without the bound the
+// scheduler trades a wave of occupancy for longer-lived staging registers.
+// The int16 and 2-bit packed kernels fit 128 VGPRs and run four.
 template <int FORM, bool INV, int WAVES, class Tiles>
 __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(3))) void k_mfma16(Tiles io,
                                                         const DeviceTables *__restrict__ tab)
@@ -266,10 +267,10 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(3)))
   int16_t *L = s_poly[w];
   int16_t *Pp = L + t * kStride;
   // Grid-stride over tiles.  Latency is hidden by the other resident waves
-  // (three per SIMD at this register count) and by launching several times
-  // more workgroups than are resident, which also balances load across CUs;
-  // a register prefetch of the whole next tile measured slower (it costs a
-  // wave of occupancy), and one of half a tile, which fits, no faster.
+  // (three or four per SIMD) and by launching several times more workgroups
+  // than are resident, which also balances load across CUs.  The passes are
+  // hidden: the kernel runs at the speed of its staging alone.  A register
+  // prefetch of the next tile (a wave of occupancy) measured no faster.
   const size_t ntiles = (io.npoly + P - 1) / P;
   const size_t stride = (size_t)gridDim.x * WAVES;
   size_t tile = (size_t)blockIdx.x * WAVES + w;
