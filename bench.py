@@ -380,6 +380,32 @@ def bench_host(tk, torch, npoly=1 << 20, reps=3):
     p.close()
     res["pageable_default"] = {"ms": t * 1e3, "poly_s": npoly / t,
                                "GBs_each_way": npoly * 512 / t / 1e9}
+
+    # PCIe reference: the same bytes between the same pinned buffers and
+    # device memory with no kernel, one way at a time and both ways at once
+    # (two streams): the bound the pipeline's each-way rate is read against.
+    dev_a = torch.empty(hin.shape, dtype=torch.int16, device="cuda:0")
+    dev_b = torch.empty(hin.shape, dtype=torch.int16, device="cuda:0")
+    s_in, s_out = torch.cuda.Stream(), torch.cuda.Stream()
+
+    def copies(h2d, d2h):
+        best = 1e30
+        for _ in range(reps + 1):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            if h2d:
+                with torch.cuda.stream(s_in):
+                    dev_a.copy_(hin, non_blocking=True)
+            if d2h:
+                with torch.cuda.stream(s_out):
+                    hout.copy_(dev_b, non_blocking=True)
+            torch.cuda.synchronize()
+            best = min(best, time.perf_counter() - t0)
+        return npoly * 512 / best / 1e9
+
+    res["pcie_reference_GBs"] = {"h2d_only": copies(True, False), "d2h_only": copies(False, True),
+                                 "both_each_way": copies(True, True)}
+    del dev_a, dev_b
     return res
 
 

@@ -142,12 +142,14 @@ int matvec_ntt_fused(int16_t *t, const int16_t *A_hat, const int16_t *s, int k, 
 /* ---- host-resident batches (PCIe streaming) ---------------------------- */
 
 /* The reference calls ntt()/invntt() per poly on host memory (ref/ntt.h:14-15,
- * ntt_bench.cu:947-956).  A host pipeline streams a host batch through the
- * device in chunks of chunk_polys: each chunk is copied in, transformed in
- * place and copied out on its own stream, so up to depth chunks are in
- * flight and the copies of both directions overlap the kernels.  Host
- * buffers from hipHostMalloc (or torch pin_memory) are moved by DMA
- * directly; pageable buffers go through the HIP runtime's staging (slower).
+ * ntt_bench.cu:947-956).  A host pipeline runs a host batch on the device:
+ * - if in and out each lie wholly inside one pinned allocation
+ *   (hipHostMalloc, torch pin_memory), the transform kernel reads and writes
+ *   them directly over PCIe (both link directions at once, nothing staged);
+ * - otherwise the batch streams through the device in chunks of chunk_polys:
+ *   each chunk is copied in, transformed in place and copied out on its own
+ *   stream, up to depth chunks in flight; pageable buffers go through the
+ *   HIP runtime's staging (slower).
  * Creation allocates depth device buffers and streams on the current
  * device; run blocks until out is written, on the creating device only. */
 #define TK_HOST_MAX_DEPTH 4

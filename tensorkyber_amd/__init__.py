@@ -302,9 +302,11 @@ def matvec_packed(A_hat, s_packed, k: int, bits: int, out=None, stream=None):
 
 
 class HostPipeline:
-    """Streams host-resident poly batches through the device in chunks
-    (tk_host_pipeline_*).  Buffers: numpy int16 arrays or CPU torch tensors
-    (pinned ones, torch.empty(..., pin_memory=True), take the DMA path)."""
+    """Transforms host-resident poly batches (tk_host_pipeline_*).  Buffers:
+    numpy int16 arrays or CPU torch tensors.  Pinned ones (torch.empty(...,
+    pin_memory=True)) are read and written by the transform kernel over PCIe;
+    others stream through the device in chunks of `chunk_polys`, `depth`
+    chunks in flight."""
 
     def __init__(self, chunk_polys: int = 16384, depth: int = 3):
         self._p = ctypes.c_void_p()

@@ -1,12 +1,20 @@
 // Host-resident batches: the reference's own calling shape is ntt(r) on host
-// memory (ref/ntt.h:14-15, called per poly from ntt_bench.cu:947-956).  A
-// pipeline streams a host batch through the device in chunks: chunk c is
-// copied host -> device, transformed in place and copied back on stream
-// c % depth, so with depth >= 2 the copies of one chunk overlap the kernel
-// and the opposite-direction copy of the others (the two DMA directions run
-// concurrently).  Pinned host buffers (hipHostMalloc, torch pin_memory) are
-// read and written by DMA directly; pageable buffers go through the HIP
-// runtime's own staging and are slower.
+// memory (ref/ntt.h:14-15, called per poly from ntt_bench.cu:947-956).
+//
+// Pinned host buffers (hipHostMalloc, torch pin_memory) are transformed by
+// the kernel itself, reading its input and writing its output over PCIe:
+// both link directions carry data at once and nothing is staged.  Copy
+// engines serialise opposite-direction copies on this device (two
+// 512 MiB copies on separate streams move 29 GB/s each way, one alone 57).
+// A buffer qualifies only if the whole byte range lies inside one pinned
+// allocation (hipMemGetAddressRange), so the kernel never touches pageable
+// memory.
+//
+// Other buffers stream through the device in chunks: chunk c is copied
+// host -> device, transformed in place and copied back on stream c % depth,
+// so with depth >= 2 the copies of one chunk overlap the kernel and the
+// copies of the others.  Pageable buffers go through the HIP runtime's own
+// staging and are slower.
 #include <hip/hip_runtime.h>
 
 #include <mutex>
@@ -34,6 +42,31 @@ void pipeline_free(tk_host_pipeline *p)
       (void)hipFree(p->dev[i]);
   }
   delete p;
+}
+
+// The device address of [p, p + bytes) if that range lies inside one pinned
+// host allocation, else nullptr.  The queries fail for pageable memory and
+// leave the runtime's last error set; it is cleared, so that the next launch
+// check does not report it.
+void *pinned_span(const void *p, size_t bytes)
+{
+  hipPointerAttribute_t a{};
+  if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  if (a.type != hipMemoryTypeHost || !a.devicePointer)
+    return nullptr;
+  hipDeviceptr_t base = nullptr;
+  size_t size = 0;
+  if (hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)a.devicePointer) != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  const uintptr_t b = (uintptr_t)base, d = (uintptr_t)a.devicePointer;
+  if (d < b || d - b > size || size - (d - b) < bytes)
+    return nullptr;
+  return a.devicePointer;
 }
 
 // Default pipeline per device for the ntt_batch_host convenience calls.
@@ -108,6 +141,13 @@ int tk_host_pipeline_run(tk_host_pipeline *p, int16_t *out, const int16_t *in, s
   if (hipGetDevice(&cur) != hipSuccess || cur != p->device)
     return TK_ENODEV;
   std::lock_guard<std::mutex> lk(p->mu);
+  void *din = pinned_span(in, bytes), *dout = din ? pinned_span(out, bytes) : nullptr;
+  if (din && dout) {
+    rc = tk::transform(direction != 0, static_cast<int16_t *>(dout),
+                       static_cast<const int16_t *>(din), npoly, m, f, p->stream[0]);
+    const int src = tk::record_hip(hipStreamSynchronize(p->stream[0]));
+    return rc ? rc : src;
+  }
   const size_t nchunks = (npoly + p->chunk - 1) / p->chunk;
   for (size_t c = 0; c < nchunks && rc == TK_OK; c++) {
     const int slot = (int)(c % (size_t)p->depth);

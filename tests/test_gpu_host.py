@@ -65,3 +65,32 @@ def test_pipeline_rejects_bad_runs(tk, gpu):
     assert L.tk_host_pipeline_run(p._p, x.ctypes.data, x.ctypes.data, 4, 0, tk.METHOD_MFMA,
                                   tk.FORM_REF) == tk.ENOTSUP
     p.close()
+
+
+@pytest.mark.parametrize("method", ["mfma16", "mfma32", "butterfly"])
+@pytest.mark.parametrize("npoly", [1, 17, 33, 20000])
+def test_pinned_buffers_transformed_over_pcie(tk, gpu, oracle, method, npoly):
+    """Pinned buffers are read and written by the transform kernel itself
+    (no staging): out of place, in place, and a view inside a larger pinned
+    allocation; then a pinned input with a pageable output takes the copy
+    path."""
+    import torch
+    x = tk.wl_uniform(npoly * N, 530 + npoly, 0, 3329).reshape(npoly, N)
+    want = canon(oracle.ntt(x))
+    p = tk.HostPipeline(4096, 2)
+    hin = torch.from_numpy(x).pin_memory()
+    hout = torch.empty((npoly, N), dtype=torch.int16, pin_memory=True)
+    p.run(hin, hout, direction=0, method=method)
+    assert np.array_equal(hout.numpy(), want), "out of place"
+    p.run(hout, direction=1, method=method)  # in place
+    assert np.array_equal(hout.numpy(), centred(oracle.invntt(want))), "in place"
+    big = torch.full((npoly + 10, N), 77, dtype=torch.int16).pin_memory()
+    big[5:5 + npoly] = torch.from_numpy(x)
+    p.run(big[5:5 + npoly], direction=0, method=method)
+    host = big.numpy()
+    assert np.array_equal(host[5:5 + npoly], want), "view inside a pinned allocation"
+    assert (host[:5] == 77).all() and (host[5 + npoly:] == 77).all(), "written outside the view"
+    pageable = np.empty_like(x)
+    p.run(hin, pageable, direction=0, method=method)
+    assert np.array_equal(pageable, want), "pinned in, pageable out"
+    p.close()
