@@ -308,7 +308,7 @@ class HostPipeline:
     others stream through the device in chunks of `chunk_polys`, `depth`
     chunks in flight."""
 
-    def __init__(self, chunk_polys: int = 16384, depth: int = 3):
+    def __init__(self, chunk_polys: int = 65536, depth: int = 2):
         self._p = ctypes.c_void_p()
         _check(lib().tk_host_pipeline_create(ctypes.byref(self._p), chunk_polys, depth),
                "tk_host_pipeline_create")

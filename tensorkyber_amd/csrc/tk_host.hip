@@ -10,24 +10,119 @@
 // allocation (hipMemGetAddressRange), so the kernel never touches pageable
 // memory.
 //
-// Other buffers stream through the device in chunks: chunk c is copied
-// host -> device, transformed in place and copied back on stream c % depth,
-// so with depth >= 2 the copies of one chunk overlap the kernel and the
-// copies of the others.  Pageable buffers go through the HIP runtime's own
-// staging and are slower.
+// Other buffers (pageable memory, which the device cannot address) are
+// staged by the pipeline itself: worker threads copy chunk c into pinned
+// staging buffer c % depth, the kernel transforms it in place over PCIe on
+// stream c % depth, and the result is copied back once that stream is done,
+// while later chunks are being staged and transformed.  The HIP runtime's
+// own pageable copies are serial and reach a quarter of that rate.
 #include <hip/hip_runtime.h>
+#include <string.h>
 
+#include <algorithm>
+#include <condition_variable>
+#include <functional>
 #include <mutex>
+#include <new>
+#include <thread>
+#include <vector>
 
 #include "tk_state.h"
 #include "tkyber.h"
+
+namespace {
+
+// A fixed set of threads running one job at a time, each thread on its own
+// share (the caller runs share 0): the copies between a caller's pageable
+// buffers and a pipeline's pinned staging.
+class CopyPool {
+ public:
+  explicit CopyPool(int n) : n_(n)
+  {
+    for (int i = 1; i < n_; i++)
+      th_.emplace_back([this, i] { loop(i); });
+  }
+  ~CopyPool()
+  {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    for (auto &t : th_)
+      t.join();
+  }
+  int size() const { return n_; }
+  // f(0) .. f(size() - 1); returns when every share is done.
+  void run(const std::function<void(int)> &f)
+  {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      job_ = &f;
+      left_ = n_ - 1;
+      ++gen_;
+    }
+    cv_.notify_all();
+    f(0);
+    std::unique_lock<std::mutex> lk(mu_);
+    done_.wait(lk, [this] { return left_ == 0; });
+    job_ = nullptr;
+  }
+
+ private:
+  void loop(int i)
+  {
+    unsigned seen = 0;
+    for (;;) {
+      const std::function<void(int)> *f = nullptr;
+      {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+        if (stop_)
+          return;
+        seen = gen_;
+        f = job_;
+      }
+      (*f)(i);
+      std::lock_guard<std::mutex> lk(mu_);
+      if (--left_ == 0)
+        done_.notify_one();
+    }
+  }
+  int n_;
+  std::vector<std::thread> th_;
+  std::mutex mu_;
+  std::condition_variable cv_, done_;
+  const std::function<void(int)> *job_ = nullptr;
+  int left_ = 0;
+  unsigned gen_ = 0;
+  bool stop_ = false;
+};
+
+// memcpy of `bytes` split into page-aligned shares over the pool.
+void parallel_copy(CopyPool &pool, void *dst, const void *src, size_t bytes)
+{
+  const size_t n = (size_t)pool.size();
+  const size_t share = ((bytes + n - 1) / n + 4095) & ~(size_t)4095;
+  pool.run([&](int i) {
+    const size_t lo = std::min(bytes, (size_t)i * share), hi = std::min(bytes, lo + share);
+    if (hi > lo)
+      memcpy(static_cast<char *>(dst) + lo, static_cast<const char *>(src) + lo, hi - lo);
+  });
+}
+
+constexpr int kCopyThreads = 8;
+
+}  // namespace
 
 struct tk_host_pipeline {
   int device = -1;
   size_t chunk = 0;
   int depth = 0;
-  int16_t *dev[TK_HOST_MAX_DEPTH] = {};
+  int16_t *stage[TK_HOST_MAX_DEPTH] = {};   // pinned host staging, chunk polys each
+  int16_t *dstage[TK_HOST_MAX_DEPTH] = {};  // their device addresses
   hipStream_t stream[TK_HOST_MAX_DEPTH] = {};
+  CopyPool *pool = nullptr;
   std::mutex mu;  // one run at a time per pipeline
 };
 
@@ -38,9 +133,10 @@ void pipeline_free(tk_host_pipeline *p)
   for (int i = 0; i < TK_HOST_MAX_DEPTH; i++) {
     if (p->stream[i])
       (void)hipStreamDestroy(p->stream[i]);
-    if (p->dev[i])
-      (void)hipFree(p->dev[i]);
+    if (p->stage[i])
+      (void)hipHostFree(p->stage[i]);
   }
+  delete p->pool;
   delete p;
 }
 
@@ -80,7 +176,8 @@ int default_pipeline(tk_host_pipeline **out)
     return TK_ENODEV;
   std::lock_guard<std::mutex> lk(g_default_mu);
   if (!g_default[dev]) {
-    int rc = tk_host_pipeline_create(&g_default[dev], TK_HOST_DEFAULT_CHUNK, 3);
+    int rc = tk_host_pipeline_create(&g_default[dev], TK_HOST_DEFAULT_CHUNK,
+                                     TK_HOST_DEFAULT_DEPTH);
     if (rc)
       return rc;
   }
@@ -102,12 +199,18 @@ int tk_host_pipeline_create(tk_host_pipeline **out, size_t chunk_polys, int dept
   int rc = tk::init_device(&st);
   if (rc)
     return rc;
-  tk_host_pipeline *p = new tk_host_pipeline();
+  tk_host_pipeline *p = new (std::nothrow) tk_host_pipeline();
+  if (!p)
+    return TK_ENOMEM;
   p->device = st->device;
   p->chunk = chunk_polys;
   p->depth = depth;
   for (int i = 0; i < depth; i++) {
-    if (hipMalloc(&p->dev[i], chunk_polys * tk::kN * sizeof(int16_t)) != hipSuccess) {
+    if (hipHostMalloc(reinterpret_cast<void **>(&p->stage[i]),
+                      chunk_polys * tk::kN * sizeof(int16_t), hipHostMallocDefault) != hipSuccess ||
+        hipHostGetDevicePointer(reinterpret_cast<void **>(&p->dstage[i]), p->stage[i], 0) !=
+            hipSuccess) {
+      (void)hipGetLastError();
       pipeline_free(p);
       return TK_ENOMEM;
     }
@@ -115,6 +218,13 @@ int tk_host_pipeline_create(tk_host_pipeline **out, size_t chunk_polys, int dept
       pipeline_free(p);
       return rc;
     }
+  }
+  try {
+    p->pool = new CopyPool(std::max(1u, std::min((unsigned)kCopyThreads,
+                                                 std::thread::hardware_concurrency())));
+  } catch (...) {
+    pipeline_free(p);
+    return TK_ENOMEM;
   }
   *out = p;
   return TK_OK;
@@ -148,25 +258,42 @@ int tk_host_pipeline_run(tk_host_pipeline *p, int16_t *out, const int16_t *in, s
     const int src = tk::record_hip(hipStreamSynchronize(p->stream[0]));
     return rc ? rc : src;
   }
+  // Staged: chunk c in staging buffer c % depth on stream c % depth; the
+  // result of chunk c - depth is copied out before that buffer is refilled.
   const size_t nchunks = (npoly + p->chunk - 1) / p->chunk;
+  auto span = [&](size_t c, size_t *off, size_t *nb) {
+    const size_t first = c * p->chunk, n = std::min(p->chunk, npoly - first);
+    *off = first * tk::kN;
+    *nb = n * tk::kN * sizeof(int16_t);
+    return n;
+  };
+  auto retire = [&](size_t c) {
+    const int slot = (int)(c % (size_t)p->depth);
+    size_t off, nb;
+    span(c, &off, &nb);
+    const int src = tk::record_hip(hipStreamSynchronize(p->stream[slot]));
+    if (src == TK_OK)
+      parallel_copy(*p->pool, out + off, p->stage[slot], nb);
+    return src;
+  };
+  size_t launched = 0;
   for (size_t c = 0; c < nchunks && rc == TK_OK; c++) {
     const int slot = (int)(c % (size_t)p->depth);
-    const size_t first = c * p->chunk;
-    const size_t n = npoly - first < p->chunk ? npoly - first : p->chunk;
-    const size_t off = first * tk::kN, nb = n * tk::kN * sizeof(int16_t);
-    hipStream_t s = p->stream[slot];
-    // Same-stream order keeps this chunk's copy-in behind the previous
-    // chunk's copy-out of the same device buffer.
-    if ((rc = tk::record_hip(hipMemcpyAsync(p->dev[slot], in + off, nb, hipMemcpyHostToDevice, s))))
+    if (c >= (size_t)p->depth && (rc = retire(c - p->depth)))
       break;
-    if ((rc = tk::transform(direction != 0, p->dev[slot], p->dev[slot], n, m, f, s)))
+    size_t off, nb;
+    const size_t n = span(c, &off, &nb);
+    parallel_copy(*p->pool, p->stage[slot], in + off, nb);
+    if ((rc = tk::transform(direction != 0, p->dstage[slot], p->dstage[slot], n, m, f,
+                            p->stream[slot])))
       break;
-    rc = tk::record_hip(hipMemcpyAsync(out + off, p->dev[slot], nb, hipMemcpyDeviceToHost, s));
+    launched = c + 1;
   }
-  for (int i = 0; i < p->depth; i++) {
-    const int src = tk::record_hip(hipStreamSynchronize(p->stream[i]));
+  const size_t first_open = launched > (size_t)p->depth ? launched - p->depth : 0;
+  for (size_t c = first_open; c < launched; c++) {
+    const int rrc = rc == TK_OK ? retire(c) : tk::record_hip(hipStreamSynchronize(p->stream[c % p->depth]));
     if (rc == TK_OK)
-      rc = src;
+      rc = rrc;
   }
   return rc;
 }
