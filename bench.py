@@ -165,8 +165,14 @@ def cpu_baseline(x):
 
     v, reps, n, t = rate(min(x.shape[0], 1 << 18), cores, 3.0)
     v1, reps1, n1, t1 = rate(min(x.shape[0], 1 << 14), 1, 2.0)
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next(l.split(":", 1)[1].strip() for l in f if l.startswith("model name"))
+    except (OSError, StopIteration):
+        pass
     return {"value": v, "unit": "poly/s", "cores": cores, "kind": kind,
-            "one_core_poly_s": v1,
+            "one_core_poly_s": v1, "cpu_model": model, "nproc": os.cpu_count(),
             "sample": f"{reps} x {n} polys of the same workload, ntt() then invntt() per "
                       f"poly, OpenMP over polys on {cores} host threads, {t:.2f} s wall "
                       f"({t * cores:.1f} core-s); one thread: {reps1} x {n1} polys, "
