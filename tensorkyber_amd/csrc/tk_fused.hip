@@ -24,9 +24,9 @@
 //   encrypt: k+1 waves and rows: u_i from row i of the transposed matrix,
 //            v from t_hat; step 3 inverse-transforms and adds e1_i, or e2 and
 //            m;
-//   decrypt: one row s_hat . u_hat, formed as k partial rows (wave j:
-//            s_hat_j o u_hat_j) summed by wave 0; step 3 inverse-transforms
-//            and writes w = reduce(v - row).
+//   decrypt: w = reduce(v - invntt(s_hat . ntt(u))) has one row, so it
+//            runs in its own kernel (k_decrypt below): one wave per tile
+//            forms the row alone and needs no workgroup barrier.
 // The adds are exact mod q.  The reference's poly_add / poly_sub are int16
 // adds on invntt's (-q, q) representatives; they agree with these wherever
 // they do not overflow int16, which holds for every ML-KEM input (noise of
@@ -138,7 +138,7 @@ constexpr int fused_waves()
 template <int MODE, int K>
 constexpr int fused_rows()
 {
-  return MODE == kFusedEncrypt ? K + 1 : (MODE == kFusedDecrypt ? 1 : K);
+  return MODE == kFusedEncrypt ? K + 1 : K;
 }
 
 // Buffer roles of FusedIo / FusedInst, in FusedInst's order.
@@ -266,7 +266,7 @@ __device__ __forceinline__ void role_out(const int16_t *slab, const RT &rt, int 
     rt.store(it, sub, v[it]);
 }
 
-// K: module rank; MODE: kFusedMatvec / Keygen / Encrypt / Decrypt (FusedIo
+// K: module rank; MODE: kFusedMatvec / Keygen / Encrypt (FusedIo
 // documents the buffers of each); SBITS = 0: the transformed vector holds
 // int16 polys [ninst][K][256]; 2, 3, 4: the packed small-coefficient form
 // [ninst][K][packed bytes].  TABLE: each instance's buffers come from the
@@ -410,41 +410,7 @@ __global__ __launch_bounds__((64 * fused_waves<MODE, K>())) void k_fused(
     // matrix loads of the next chunk are in flight while this one is summed;
     // the scheduling barrier keeps the compiler from hoisting more of them.
     uint32_t res[P / 2][4];
-    if constexpr (MODE == kFusedDecrypt) {
-      // The one output row is split across the waves: wave w forms the
-      // partial s_hat_w o u_hat_w against its own slab and then overwrites
-      // that slab with it (no other wave reads it); wave 0 sums the partials
-      // after the barrier.
-      const auto um = role(rMat);
-      auto load_part = [&](v4i &a, int it) { a = um.load(it, w); };
-      v4i a_cur, a_nxt;
-      load_part(a_cur, 0);
-#pragma unroll
-      for (int it = 0; it < P / 2; it++) {
-        const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
-        if (it + 1 < P / 2)
-          load_part(a_nxt, it + 1);
-        const v4i b = *reinterpret_cast<const v4i *>(slab + pp * kStride + ch * 8);
-        const uint2 gg = *reinterpret_cast<const uint2 *>(&s_g[w][pp][2 * ch]);
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-          const uint32_t bu = (uint32_t)b[u], gs = u < 2 ? gg.x : gg.y;
-          const uint32_t x = __builtin_amdgcn_perm(gs, bu, (u & 1) ? 0x07060100u : 0x05040100u);
-          const uint32_t y = __builtin_amdgcn_alignbit(bu, bu, 16);
-          res[it][u] = pack2(red32((uint32_t)dot2((uint32_t)a_cur[u], x, 0) + kSumOffset),
-                             red32((uint32_t)dot2((uint32_t)a_cur[u], y, 0) + kSumOffset));
-        }
-        a_cur = a_nxt;
-        __builtin_amdgcn_sched_barrier(0);
-      }
-      wave_sync();  // this wave's reads of its slab are done
-#pragma unroll
-      for (int it = 0; it < P / 2; it++) {
-        const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
-        *reinterpret_cast<v4i *>(slab + pp * kStride + ch * 8) =
-            v4i{(int)res[it][0], (int)res[it][1], (int)res[it][2], (int)res[it][3]};
-      }
-    } else if (w < R) {
+    if (w < R) {
       const auto mr = role(row_role);
       auto load_row = [&](v4i *a, int it) {
 #pragma unroll
@@ -484,26 +450,6 @@ __global__ __launch_bounds__((64 * fused_waves<MODE, K>())) void k_fused(
     }
     __syncthreads();  // every wave is done reading the slabs
 
-    if constexpr (MODE == kFusedDecrypt) {
-      // Wave 0 sums the partials (each in [0, 2q), the sum below 8q) into
-      // its slab; the barrier then frees the other slabs for the next tile.
-      if (w == 0) {
-#pragma unroll
-        for (int it = 0; it < P / 2; it++) {
-          const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
-          v4i acc = *reinterpret_cast<const v4i *>(s_slab[0] + pp * kStride + ch * 8);
-#pragma unroll
-          for (int j = 1; j < K; j++)
-            acc = add16(acc, *reinterpret_cast<const v4i *>(s_slab[j] + pp * kStride + ch * 8));
-          res[it][0] = (uint32_t)acc[0];
-          res[it][1] = (uint32_t)acc[1];
-          res[it][2] = (uint32_t)acc[2];
-          res[it][3] = (uint32_t)acc[3];
-        }
-      }
-      __syncthreads();
-    }
-
     // 3. Output row w.
     if (w < R) {
       if constexpr (MODE == kFusedKeygen) {
@@ -535,8 +481,8 @@ __global__ __launch_bounds__((64 * fused_waves<MODE, K>())) void k_fused(
         if constexpr (MODE == kFusedMatvec) {
           role_out<P>(slab, role(rOut0), w, lane);
         } else {
-          // Addends of the output row: e1_w (a u row), e2 and m (the v row),
-          // v (decrypt).  All loads go out before the first sum.
+          // Addends of the output row: e1_w (a u row), e2 and m (the v row).
+          // All loads go out before the first sum.
           const bool vrow = MODE == kFusedEncrypt && w == K;
           const int sub = (MODE == kFusedEncrypt && !vrow) ? w : 0;
           const auto ya = role(vrow ? rAdd1 : rAdd0);
@@ -553,14 +499,187 @@ __global__ __launch_bounds__((64 * fused_waves<MODE, K>())) void k_fused(
           for (int it = 0; it < P / 2; it++) {
             const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
             const v4i x = *reinterpret_cast<const v4i *>(slab + pp * kStride + ch * 8);
-            if constexpr (MODE == kFusedEncrypt)  // u_w = reduce(x + e1_w), v = reduce(x + e2 + m)
-              oa.store(it, sub, centred_sum<1>(x, y[it], z[it]));
-            else  // decrypt: w = reduce(v - x)
-              oa.store(it, sub, centred_sum<-1>(x, y[it], zero));
+            // u_w = reduce(x + e1_w), v = reduce(x + e2 + m)
+            oa.store(it, sub, centred_sum<1>(x, y[it], z[it]));
           }
         }
       }
       wave_sync();
+    }
+  }
+}
+
+constexpr int kDecryptWaves = 4;
+
+// Decrypt, one wave per tile: w = reduce(v - invntt(sum_j s_hat_j o
+// ntt(u_j))).  Decrypt has one output row, so a wave forms it alone: for
+// each j it stages u_j of its 16 instances into its slab, forward-transforms
+// it and adds s_hat_j o u_hat_j (s_hat_j streamed from HBM) into the row it
+// holds in registers, packed; then it inverse-transforms the row in the slab
+// and writes w.  The waves of a workgroup share only the bias tables, so the
+// tile loop has no workgroup barrier.  Each slab value enters one product,
+// so gamma * b1 is formed where it is used rather than tabulated.
+//
+// The row's partial sums are reduced per j to [0, 2q) and added as packed
+// int16 pairs (below 8q for K <= 4: no carry between the halves); the
+// inverse's first pass takes inputs below 8q, as the fused kernel's rows.
+template <int K, bool TABLE>
+__global__ __launch_bounds__(64 * kDecryptWaves)
+__attribute__((amdgpu_waves_per_eu(TABLE ? 3 : 4))) void k_decrypt(const FusedIo io, const DeviceTables *__restrict__ tab)
+{
+  constexpr int P = kTileInst, W = kDecryptWaves;
+  __shared__ const int16_t *s_ip[TABLE ? W : 1][TABLE ? P : 1][8];
+  __shared__ __align__(16) int16_t s_slab[W][P * kStride];
+  __shared__ __align__(16) int32_t s_bf2[16][16];
+  __shared__ __align__(16) int32_t s_bi2[16][16];
+  const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int lane = threadIdx.x & 63, g = lane >> 4, tt = lane & 15;
+
+  for (int i = threadIdx.x; i < 256; i += 64 * W) {
+    s_bf2[i >> 4][i & 15] = tab->bias_fwd2[i >> 4][i & 15];
+    s_bi2[i >> 4][i & 15] = tab->bias_inv2[i >> 4][i & 15];
+  }
+  __syncthreads();
+
+  const long f1lo = ld_a(tab->m16_fwd1[0], 0, lane), f1hi = ld_a(tab->m16_fwd1[0], 1, lane);
+  const long i1lo = ld_a(tab->m16_inv1p[0], 0, lane), i1hi = ld_a(tab->m16_inv1p[0], 1, lane);
+  const v4i fb1 = *reinterpret_cast<const v4i *>(&tab->bias_fwd1[0][4 * g]);
+  const v4i ib1 = *reinterpret_cast<const v4i *>(&tab->bias_inv1p[0][4 * g]);
+  const __amdgpu_buffer_rsrc_t trs = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<DeviceTables *>(tab), 0, (int)sizeof(DeviceTables), kBufferRsrcWord3);
+  const int loff = lane * 8;
+  auto mat = [&](int off) {
+    return __builtin_bit_cast(long, __builtin_amdgcn_raw_buffer_load_b64(trs, loff, off, 0));
+  };
+  constexpr int kF2 = (int)offsetof(DeviceTables, m16_fwd2), kI2 = (int)offsetof(DeviceTables, m16_inv2);
+  auto f2lo = [&](int m) { return mat(kF2 + m * 1024); };
+  auto f2hi = [&](int m) { return mat(kF2 + m * 1024 + 512); };
+  auto i2lo = [&](int m) { return mat(kI2 + m * 1024); };
+  auto i2hi = [&](int m) { return mat(kI2 + m * 1024 + 512); };
+  // Lane (., ch) holds coefficients 8ch .. 8ch+7 of a poly: pairs u = 0..3,
+  // pair u in group 2ch + u/2 with gamma for even u and q - gamma for odd u.
+  uint32_t gam[4];
+  {
+    const uint32_t ga = tab->gamma[2 * (lane & 31)], gb = tab->gamma[2 * (lane & 31) + 1];
+    gam[0] = ga;
+    gam[1] = (uint32_t)kQ - ga;
+    gam[2] = gb;
+    gam[3] = (uint32_t)kQ - gb;
+  }
+
+  int16_t *const slab = s_slab[w];
+  int16_t *const pslab = slab + tt * kStride;
+  size_t i0 = 0;
+  int nvalid = 0;
+  auto role = [&](int rl) {
+    if constexpr (TABLE)
+      return RoleTile<true>{s_ip[w], rl, lane};
+    else
+      return RoleTile<false>(role_base(io, rl), i0, nvalid,
+                             (int)role_stride<kFusedDecrypt, K>(rl), lane);
+  };
+
+  const size_t ntiles = (io.ninst + P - 1) / P, nwaves = (size_t)gridDim.x * W;
+  for (size_t tile = (size_t)blockIdx.x * W + w; tile < ntiles; tile += nwaves) {
+    i0 = tile * P;
+    nvalid = (int)(io.ninst - i0 < (size_t)P ? io.ninst - i0 : P);
+    if constexpr (TABLE) {
+      // The wave's own rows of pointers; its reads of the previous tile's
+      // rows were issued before these writes.
+      wave_sync();
+      for (int idx = lane; idx < P * 8; idx += 64) {
+        const int pp = idx >> 3, rl = idx & 7;
+        s_ip[w][pp][rl] = pp < nvalid
+                              ? reinterpret_cast<const int16_t *const *>(io.inst + i0 + pp)[rl]
+                              : nullptr;
+      }
+      wave_sync();
+    }
+
+    // Software-pipelined over the tile's HBM reads: u_{j+1} is in flight
+    // during the product with s_hat_j, whose loads all go out at once.
+    const auto um = role(rX);
+    const auto sm = role(rMat);
+    uint32_t acc[P / 2][4];
+    v4i un[P / 2];
+#pragma unroll
+    for (int it = 0; it < P / 2; it++) {
+      acc[it][0] = acc[it][1] = acc[it][2] = acc[it][3] = 0;
+      un[it] = um.load(it, 0);
+    }
+    // One j: u_hat_j = ntt(u_j) (canonical, in the slab), then acc +=
+    // s_hat_j o u_hat_j with chunk `it` of u_{j+1} loaded into the registers
+    // chunk `it` of s_hat_j frees.  The last j is a separate instance that
+    // loads nothing: a conditional load would keep the old chunk values live
+    // across the forward transform.
+    auto step = [&](int j, auto last_c) {
+      constexpr bool kLast = decltype(last_c)::value;
+      wave_sync();
+#pragma unroll
+      for (int it = 0; it < P / 2; it++) {
+        const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
+        *reinterpret_cast<v4i *>(slab + pp * kStride + ch * 8) = un[it];
+      }
+      wave_sync();
+      pass_m16<kFormInter>(pslab, g, f1lo, f1hi, fb1);
+      wave_sync();
+      pass_c16<kFormCanon>(pslab, g, f2lo, f2hi, s_bf2);
+      wave_sync();
+      v4i sn[P / 2];
+#pragma unroll
+      for (int it = 0; it < P / 2; it++)
+        sn[it] = sm.load(it, j);
+#pragma unroll
+      for (int it = 0; it < P / 2; it++) {
+        const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
+        const v4i b = *reinterpret_cast<const v4i *>(slab + pp * kStride + ch * 8);
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const uint32_t bu = (uint32_t)b[u];
+          const uint32_t gb1 = red24((bu >> 16) * gam[u]);
+          const uint32_t x = __builtin_amdgcn_perm(gb1, bu, 0x05040100u);
+          const uint32_t y = __builtin_amdgcn_alignbit(bu, bu, 16);
+          const uint32_t r = pack2(red32((uint32_t)dot2((uint32_t)sn[it][u], x, 0) + kSumOffset),
+                                   red32((uint32_t)dot2((uint32_t)sn[it][u], y, 0) + kSumOffset));
+          acc[it][u] += r;
+        }
+        if constexpr (!kLast)
+          un[it] = um.load(it, j + 1);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    };
+#pragma unroll 1
+    for (int j = 0; j < K - 1; j++)
+      step(j, std::false_type{});
+    step(K - 1, std::true_type{});
+
+    // The row into the slab (its reads above are done), inverse-transformed.
+    wave_sync();
+#pragma unroll
+    for (int it = 0; it < P / 2; it++) {
+      const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
+      *reinterpret_cast<v4i *>(slab + pp * kStride + ch * 8) =
+          v4i{(int)acc[it][0], (int)acc[it][1], (int)acc[it][2], (int)acc[it][3]};
+    }
+    wave_sync();
+    pass_c16<kFormInter>(pslab, g, i2lo, i2hi, s_bi2);
+    wave_sync();
+    pass_m16<kFormCentred>(pslab, g, i1lo, i1hi, ib1);
+    wave_sync();
+
+    // w = reduce(v - row); every load of v goes out before the first store
+    // (w may alias v).
+    const auto va = role(rAdd0);
+    const auto oa = role(rOut0);
+    const v4i zero = {0, 0, 0, 0};
+#pragma unroll
+    for (int it = 0; it < P / 2; it++)
+      un[it] = va.load(it, 0);
+#pragma unroll
+    for (int it = 0; it < P / 2; it++) {
+      const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
+      const v4i x = *reinterpret_cast<const v4i *>(slab + pp * kStride + ch * 8);
+      oa.store(it, 0, centred_sum<-1>(x, un[it], zero));
     }
   }
 }
@@ -611,8 +730,26 @@ int launch_fused(int k, int mode, int s_bits, const FusedIo &io, const DeviceTab
       return s_bits ? -1 : by_table(integral_constant<int, kFusedKeygen>{});
     case kFusedEncrypt:
       return s_bits ? -1 : by_table(integral_constant<int, kFusedEncrypt>{});
-    case kFusedDecrypt:
-      return s_bits ? -1 : by_table(integral_constant<int, kFusedDecrypt>{});
+    case kFusedDecrypt: {
+      if (s_bits)
+        return -1;
+      auto dec = [&](auto kernel) {
+        const size_t cap = (size_t)ncu * resident_per_cu((const void *)kernel, 64 * kDecryptWaves);
+        const size_t wgs = (ntiles + kDecryptWaves - 1) / kDecryptWaves;
+        hipLaunchKernelGGL(kernel, dim3((unsigned)(wgs < cap ? wgs : cap)),
+                           dim3(64 * kDecryptWaves), 0, stream, io, tab);
+        return 0;
+      };
+      switch (k * 2 + (io.inst ? 1 : 0)) {
+        case 4: return dec(k_decrypt<2, false>);
+        case 5: return dec(k_decrypt<2, true>);
+        case 6: return dec(k_decrypt<3, false>);
+        case 7: return dec(k_decrypt<3, true>);
+        case 8: return dec(k_decrypt<4, false>);
+        case 9: return dec(k_decrypt<4, true>);
+        default: return -1;
+      }
+    }
     default:
       return -1;
   }
