@@ -161,7 +161,8 @@ typedef struct tk_host_pipeline tk_host_pipeline;
 int tk_host_pipeline_create(tk_host_pipeline **pipeline, size_t chunk_polys, int depth);
 /* direction 0: forward (default form canonical), 1: inverse (centred);
  * method and form as for ntt_batch_ex; out may equal in (no partial
- * overlap).  One run at a time per pipeline (others wait). */
+ * overlap).  in and out are host memory: device memory gives TK_EINVAL.
+ * One run at a time per pipeline (others wait). */
 int tk_host_pipeline_run(tk_host_pipeline *pipeline, int16_t *out, const int16_t *in,
                          size_t npoly, int direction, int method, int form);
 int tk_host_pipeline_destroy(tk_host_pipeline *pipeline);

@@ -37,21 +37,18 @@ namespace {
 // buffers and a pipeline's pinned staging.
 class CopyPool {
  public:
+  // Throws if a thread cannot be started, after joining those that were.
   explicit CopyPool(int n) : n_(n)
   {
-    for (int i = 1; i < n_; i++)
-      th_.emplace_back([this, i] { loop(i); });
-  }
-  ~CopyPool()
-  {
-    {
-      std::lock_guard<std::mutex> lk(mu_);
-      stop_ = true;
+    try {
+      for (int i = 1; i < n_; i++)
+        th_.emplace_back([this, i] { loop(i); });
+    } catch (...) {
+      stop_all();
+      throw;
     }
-    cv_.notify_all();
-    for (auto &t : th_)
-      t.join();
   }
+  ~CopyPool() { stop_all(); }
   int size() const { return n_; }
   // f(0) .. f(size() - 1); returns when every share is done.
   void run(const std::function<void(int)> &f)
@@ -70,6 +67,17 @@ class CopyPool {
   }
 
  private:
+  void stop_all()
+  {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    for (auto &t : th_)
+      t.join();
+    th_.clear();
+  }
   void loop(int i)
   {
     unsigned seen = 0;
@@ -140,29 +148,40 @@ void pipeline_free(tk_host_pipeline *p)
   delete p;
 }
 
-// The device address of [p, p + bytes) if that range lies inside one pinned
-// host allocation, else nullptr.  The queries fail for pageable memory and
-// leave the runtime's last error set; it is cleared, so that the next launch
-// check does not report it.
-void *pinned_span(const void *p, size_t bytes)
+// Where the bytes [p, p + bytes) live:
+//   kPinned  inside one pinned host allocation; *dev is their device address;
+//   kDevice  device memory, which the host copies of the staged path cannot
+//            touch (the run is rejected);
+//   kOther   anything else, taken to be host memory the device cannot
+//            address (pageable), or a pinned range that overruns its
+//            allocation.
+// The queries fail for pageable memory and leave the runtime's last error
+// set; it is cleared, so that the next launch check does not report it.
+enum SpanKind { kOther, kPinned, kDevice };
+
+SpanKind host_span(const void *p, size_t bytes, void **dev)
 {
+  *dev = nullptr;
   hipPointerAttribute_t a{};
   if (hipPointerGetAttributes(&a, p) != hipSuccess) {
     (void)hipGetLastError();
-    return nullptr;
+    return kOther;
   }
+  if (a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeArray)
+    return kDevice;
   if (a.type != hipMemoryTypeHost || !a.devicePointer)
-    return nullptr;
+    return kOther;
   hipDeviceptr_t base = nullptr;
   size_t size = 0;
   if (hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)a.devicePointer) != hipSuccess) {
     (void)hipGetLastError();
-    return nullptr;
+    return kOther;
   }
   const uintptr_t b = (uintptr_t)base, d = (uintptr_t)a.devicePointer;
   if (d < b || d - b > size || size - (d - b) < bytes)
-    return nullptr;
-  return a.devicePointer;
+    return kOther;
+  *dev = a.devicePointer;
+  return kPinned;
 }
 
 // Default pipeline per device for the ntt_batch_host convenience calls.
@@ -250,9 +269,12 @@ int tk_host_pipeline_run(tk_host_pipeline *p, int16_t *out, const int16_t *in, s
   int cur = -1;
   if (hipGetDevice(&cur) != hipSuccess || cur != p->device)
     return TK_ENODEV;
+  void *din = nullptr, *dout = nullptr;
+  const SpanKind kin = host_span(in, bytes, &din), kout = host_span(out, bytes, &dout);
+  if (kin == kDevice || kout == kDevice)
+    return TK_EINVAL;
   std::lock_guard<std::mutex> lk(p->mu);
-  void *din = pinned_span(in, bytes), *dout = din ? pinned_span(out, bytes) : nullptr;
-  if (din && dout) {
+  if (kin == kPinned && kout == kPinned) {
     rc = tk::transform(direction != 0, static_cast<int16_t *>(dout),
                        static_cast<const int16_t *>(din), npoly, m, f, p->stream[0]);
     const int src = tk::record_hip(hipStreamSynchronize(p->stream[0]));

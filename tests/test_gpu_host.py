@@ -64,6 +64,18 @@ def test_pipeline_rejects_bad_runs(tk, gpu):
     assert L.tk_host_pipeline_run(p._p, x.ctypes.data, x.ctypes.data, 4, 2, 0, 0) == tk.EINVAL
     assert L.tk_host_pipeline_run(p._p, x.ctypes.data, x.ctypes.data, 4, 0, tk.METHOD_MFMA,
                                   tk.FORM_REF) == tk.ENOTSUP
+    # device memory on either side (the staged path's host copies cannot touch it)
+    import torch
+    d = torch.zeros(8 * N, dtype=torch.int16, device="cuda:0")
+    before = d.clone()
+    assert L.tk_host_pipeline_run(p._p, d.data_ptr(), x.ctypes.data, 4, 0, 0, 0) == tk.EINVAL
+    assert L.tk_host_pipeline_run(p._p, x.ctypes.data, d.data_ptr(), 4, 0, 0, 0) == tk.EINVAL
+    assert L.tk_host_pipeline_run(p._p, d.data_ptr(), d.data_ptr(), 4, 0, 0, 0) == tk.EINVAL
+    torch.cuda.synchronize()
+    assert torch.equal(d, before)
+    # the pipeline still runs after the rejected calls
+    y = np.arange(8 * N, dtype=np.int16).reshape(8, N) % 3329
+    assert L.tk_host_pipeline_run(p._p, y.ctypes.data, y.ctypes.data, 8, 0, 0, 0) == tk.OK
     p.close()
 
 
