@@ -30,7 +30,7 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_POLY_PASS = 1024  # 512 B read + 512 B written per poly per transform
 
 
-EXTRAS = ("ragged", "matvec", "packed", "kpke", "sched", "host")
+EXTRAS = ("basemul", "ragged", "matvec", "packed", "kpke", "sched", "host")
 
 
 def parse():
@@ -227,6 +227,31 @@ def bench_ragged(tk, torch, steps):
             "rt_ms_direct_launch": ms_direct,
             "method": {3: "mfma16", 4: "mfma32"}.get(info[0].value, info[0].value),
             "oversub": info[2].value, "launch": "hipGraph replay of one persistent kernel"}
+
+
+def bench_basemul(tk, torch, steps, npoly=1 << 20):
+    """basemul_batch on 2^20 NTT-domain poly pairs (uniform [0, q), seeds 6
+    and 7): the reference's basemul on all 128 pairs of each poly, raw ref
+    values out.  HBM-bound: 2 x 512 B read + 512 B written per poly."""
+    N = 256
+    a = torch.from_numpy(tk.wl_uniform(npoly * N, 6)).to("cuda:0")
+    b = torch.from_numpy(tk.wl_uniform(npoly * N, 7)).to("cuda:0")
+    r = torch.empty_like(a)
+    stream = torch.cuda.current_stream()
+    fn = lambda: tk.basemul(a, b, out=r)  # noqa: E731
+    warm(torch, fn)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    nbytes = npoly * 3 * 512
+    return {"workload": f"{npoly} poly pairs, uniform [0,q), seeds 6 and 7", "us": ms * 1e3,
+            "poly_s": npoly / (ms / 1e3), "bytes_per_poly": 3 * 512,
+            "GBs": nbytes / (ms / 1e3) / 1e9,
+            "roofline_frac": nbytes / (ms / 1e3) / 1e9 / HBM_PEAK_GBS}
 
 
 def bench_matvec(tk, torch, steps, lat_launches=1000):
@@ -517,6 +542,7 @@ def load_traffic(kernel, npoly):
 def run_extras(tk, torch, args):
     """The sections beyond the headline, as selected by --extras."""
     sections = {
+        "basemul": ("basemul", "basemul_batch", lambda: bench_basemul(tk, torch, args.steps)),
         "ragged": ("config4_ragged", "config 4 ragged plan", lambda: bench_ragged(tk, torch, args.steps)),
         "matvec": ("config5_matvec", "config 5 fused matvec", lambda: bench_matvec(tk, torch, args.steps)),
         "packed": ("packed_input", "packed input", lambda: bench_packed(tk, torch, args.steps)),

@@ -244,8 +244,13 @@ int launch_butterfly(bool inverse, int form, int16_t *out, const int16_t *in, si
   const size_t ngroups = (npoly + kPolysPerWave - 1) / kPolysPerWave;
   const size_t need = (ngroups + kWaves - 1) / kWaves;
   const int block = 64 * kWaves;
+  // Four times the resident workgroups, each looping over its groups: on
+  // MI355X 2^20 polys take 222-226 us forward and 306-312 us inverse, against
+  // 231 and 335 us for one resident set (two: 228-232 and 315-317; eight:
+  // the same as four) and 264 and 350 us for a grid that covers the batch.
+  constexpr int kOversub = 4;
   auto go = [&](auto kernel) {
-    const size_t cap = (size_t)ncu * resident_per_cu((const void *)kernel, block);
+    const size_t cap = (size_t)ncu * resident_per_cu((const void *)kernel, block) * kOversub;
     const dim3 grid((unsigned)(need < cap ? need : cap));
     hipLaunchKernelGGL(kernel, grid, dim3(block), 0, stream, out, in, npoly, tab);
     return 0;
