@@ -43,11 +43,15 @@ struct PackedTiles {
 // multiple of the workgroups resident at once (from the occupancy
 // calculator).  Each wave strides over tiles; a factor above 1 lets the
 // dispatcher hand late workgroups to whichever CU frees up first, which
-// balances the load.  Linear and packed batches use 8 (measured 2% faster
-// than 4 for the 16-poly tiles, which compute their poly addresses from the
-// tile index); ragged plans start from 4 and let the autotuner choose.
+// balances the load.  Linear batches use 8 (measured 2% faster than 4 for
+// the 16-poly tiles, which compute their poly addresses from the tile
+// index).  Packed batches use 2: their waves load few bytes per tile, and
+// on MI355X 2^20 packed polys take 137-140 us (2 bits) and 142-145 us
+// (3 bits) at 1 or 2, against 146 and 153 us at 8 and 168 and 164 us at 32.
+// Ragged plans start from 4 and let the autotuner choose.
 constexpr int kDefaultOversub = 4;
 constexpr int kLinearOversub = 8;
+constexpr int kPackedOversub = 2;
 struct LaunchShape {
   int ncu;
   int waves;
