@@ -180,7 +180,9 @@ int transform(bool inverse, int16_t *out, const int16_t *in, size_t npoly, int m
     rc = launch_butterfly(inverse, device_form(f), out, in, npoly, st->tables, st->ncu, s);
   else
     rc = launch_mfma(m == TK_METHOD_MFMA32 ? 32 : 16, inverse, device_form(f), out, in, npoly,
-                     st->tables, LaunchShape{st->ncu, 4, kLinearOversub}, s);
+                     st->tables,
+                     LaunchShape{st->ncu, 4, m == TK_METHOD_MFMA32 ? kLinearOversub32 : kLinearOversub},
+                     s);
   if (rc)
     return TK_ENOTSUP;
   return after_launch();

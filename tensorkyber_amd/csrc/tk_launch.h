@@ -43,14 +43,16 @@ struct PackedTiles {
 // multiple of the workgroups resident at once (from the occupancy
 // calculator).  Each wave strides over tiles; a factor above 1 lets the
 // dispatcher hand late workgroups to whichever CU frees up first, which
-// balances the load.  Linear batches use 8 (measured 2% faster than 4 for
-// the 16-poly tiles, which compute their poly addresses from the tile
-// index).  Packed batches use 2: their waves load few bytes per tile, and
+// balances the load.  Linear batches use 8 for 16-poly tiles (2^20 polys:
+// 182-183 us per pass, against 190 at 4 and 192 at 2) and 12 for 32-poly
+// tiles (180-183 / 184-188 us forward / inverse, against 188-190 / 190-192
+// at 8; the inverse slows again from 16).  Packed batches use 2: their waves load few bytes per tile, and
 // on MI355X 2^20 packed polys take 137-140 us (2 bits) and 142-145 us
 // (3 bits) at 1 or 2, against 146 and 153 us at 8 and 168 and 164 us at 32.
 // Ragged plans start from 4 and let the autotuner choose.
 constexpr int kDefaultOversub = 4;
 constexpr int kLinearOversub = 8;
+constexpr int kLinearOversub32 = 12;
 constexpr int kPackedOversub = 2;
 struct LaunchShape {
   int ncu;
