@@ -4,6 +4,8 @@
   rows sampled across the whole range (both ends, around 2^22, random)
   checked against the oracle, and the round trip invntt(ntt(x)) == 2^16 x
   (mod q) checked on every coefficient, chunk by chunk, on the device.
+* basemul over 2^23 pairs: every coefficient through the Montgomery identity,
+  sampled rows against the oracle.
 * Several host threads calling the library at once, each on its own stream
   and buffers, through the transforms, the fused product and a ragged plan.
 """
@@ -48,6 +50,38 @@ def test_four_gib_batch(tk, gpu, oracle, method):
         got = y[c:c + step].to(torch.int32) % Q
         assert torch.equal(got, want), f"round trip, polys {c}.."
     del x, y
+    torch.cuda.empty_cache()
+
+
+def test_four_gib_basemul(tk, gpu, oracle):
+    """basemul_batch over 2^23 poly pairs (a grid of 2^20 workgroups): every
+    coefficient through the Montgomery identity element (b0 = 2^16 mod q,
+    b1 = 0 on every pair gives basemul(a, b) == a mod q), and sampled rows
+    with random b against the oracle."""
+    import torch
+    npoly = 1 << 23
+    g = torch.Generator(device=gpu)
+    g.manual_seed(29)
+    a = torch.randint(-32768, 32768, (npoly, N), dtype=torch.int16, device=gpu, generator=g)
+    one = torch.zeros((npoly, N), dtype=torch.int16, device=gpu)
+    one[:, 0::2] = 65536 % Q
+    print("4 GiB basemul, identity", flush=True)
+    r = tk.basemul(a, one, form="canonical")
+    step = 1 << 20
+    for c in range(0, npoly, step):
+        assert torch.equal(r[c:c + step].to(torch.int32), a[c:c + step].to(torch.int32) % Q), \
+            f"identity, polys {c}.."
+    del one
+    print("4 GiB basemul, random rows", flush=True)
+    b = torch.randint(0, Q, (npoly, N), dtype=torch.int16, device=gpu, generator=g)
+    tk.basemul(a, b, out=r)
+    rng = np.random.default_rng(7)
+    rows = np.unique(np.concatenate([[0, 1, (1 << 22) - 1, 1 << 22, npoly - 2, npoly - 1],
+                                     rng.integers(0, npoly, 2000)]))
+    ridx = torch.from_numpy(rows).to(gpu)
+    want = oracle.basemul(a.index_select(0, ridx).cpu().numpy(), b.index_select(0, ridx).cpu().numpy())
+    assert np.array_equal(r.index_select(0, ridx).cpu().numpy(), want)
+    del a, b, r
     torch.cuda.empty_cache()
 
 
