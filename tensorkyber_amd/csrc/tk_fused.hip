@@ -2,12 +2,11 @@
 // ranks k = 2..4, one kernel.  A workgroup has k waves and works on tiles of
 // 16 instances:
 //
-//   1. wave j stages s_j of the tile's 16 instances into its LDS slab,
-//      forward-transforms it (canonical) and tabulates gamma * b1 for its
-//      pairs;                                                   -- barrier --
+//   1. wave j stages s_j of the tile's 16 instances into its LDS slab and
+//      forward-transforms it (canonical);                        -- barrier --
 //   2. wave i accumulates output row i, sum_j A_hat[i][j] o s_hat[j], with
-//      A_hat streamed from HBM and every s_hat[j] read from LDS; the row is
-//      held in registers;                                       -- barrier --
+//      A_hat streamed from HBM and every s_hat[j] read from LDS (gamma * b1
+//      formed as it is read); the row is held in registers;     -- barrier --
 //   3. wave i writes its row into its own slab, inverse-transforms it and
 //      stores t_i.
 //
@@ -280,7 +279,6 @@ __global__ __launch_bounds__((64 * fused_waves<MODE, K>())) void k_fused(
   constexpr int P = kTileInst, W = fused_waves<MODE, K>(), R = fused_rows<MODE, K>();
   __shared__ const int16_t *s_ip[TABLE ? 2 : 1][TABLE ? P : 1][8];
   __shared__ __align__(16) int16_t s_slab[W][P * kStride];
-  __shared__ __align__(16) uint32_t s_g[K][P][64];  // (gamma*b1, (q-gamma)*b3) per group
   __shared__ __align__(16) int32_t s_bf2[16][16];
   __shared__ __align__(16) int32_t s_bi2[16][16];
   const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -317,8 +315,19 @@ __global__ __launch_bounds__((64 * fused_waves<MODE, K>())) void k_fused(
   auto f2hi = [&](int m) { return mat(kF2 + m * 1024 + 512); };
   auto i2lo = [&](int m) { return mat(kI2 + m * 1024); };
   auto i2hi = [&](int m) { return mat(kI2 + m * 1024 + 512); };
-  // Lane `lane` tabulates group i = lane: pairs (4i, 4i+1) and (4i+2, 4i+3).
-  const uint32_t gam = tab->gamma[lane], gam_neg = (uint32_t)kQ - gam;
+  // Lane (., ch) reads coefficients 8ch .. 8ch+7 of a poly in the row
+  // products: pairs u = 0..3, pair u in group 2ch + u/2, with gamma for even
+  // u and q - gamma for odd u.  gamma * b1 is formed where it is used: a
+  // table of it per transformed poly would cost 4 KB of LDS per wave, and
+  // with it a workgroup per CU.
+  uint32_t gin[4];
+  {
+    const uint32_t ga = tab->gamma[2 * (lane & 31)], gb = tab->gamma[2 * (lane & 31) + 1];
+    gin[0] = ga;
+    gin[1] = (uint32_t)kQ - ga;
+    gin[2] = gb;
+    gin[3] = (uint32_t)kQ - gb;
+  }
 
   int16_t *const slab = s_slab[w];
   int16_t *const pslab = slab + tt * kStride;
@@ -389,11 +398,6 @@ __global__ __launch_bounds__((64 * fused_waves<MODE, K>())) void k_fused(
         pass_c16<kFormCanon>(pslab, g, f2lo, f2hi, s_bf2);
         wave_sync();
       }
-#pragma unroll 4
-      for (int pp = 0; pp < P; pp++) {
-        const uint2 d = ld2(slab + pp * kStride + 4 * lane);
-        s_g[w][pp][lane] = pack2(red24((d.x >> 16) * gam), red24((d.y >> 16) * gam_neg));
-      }
       if constexpr (MODE == kFusedKeygen) {
         // s_hat = poly_ntt(s_w): the centred representatives.
         const auto o1 = role(rOut1);
@@ -428,11 +432,10 @@ __global__ __launch_bounds__((64 * fused_waves<MODE, K>())) void k_fused(
 #pragma unroll
         for (int j = 0; j < K; j++) {
           const v4i b = *reinterpret_cast<const v4i *>(s_slab[j] + pp * kStride + ch * 8);
-          const uint2 gg = *reinterpret_cast<const uint2 *>(&s_g[j][pp][2 * ch]);
 #pragma unroll
           for (int u = 0; u < 4; u++) {
-            const uint32_t bu = (uint32_t)b[u], gs = u < 2 ? gg.x : gg.y;
-            const uint32_t x = __builtin_amdgcn_perm(gs, bu, (u & 1) ? 0x07060100u : 0x05040100u);
+            const uint32_t bu = (uint32_t)b[u];
+            const uint32_t x = __builtin_amdgcn_perm(red24((bu >> 16) * gin[u]), bu, 0x05040100u);
             const uint32_t y = __builtin_amdgcn_alignbit(bu, bu, 16);
             r0[u] = dot2((uint32_t)a_cur[j][u], x, r0[u]);
             r1[u] = dot2((uint32_t)a_cur[j][u], y, r1[u]);
@@ -517,8 +520,8 @@ constexpr int kDecryptWaves = 4;
 // it and adds s_hat_j o u_hat_j (s_hat_j streamed from HBM) into the row it
 // holds in registers, packed; then it inverse-transforms the row in the slab
 // and writes w.  The waves of a workgroup share only the bias tables, so the
-// tile loop has no workgroup barrier.  Each slab value enters one product,
-// so gamma * b1 is formed where it is used rather than tabulated.
+// tile loop has no workgroup barrier.  gamma * b1 is formed where it is
+// used, as in the fused kernel's rows.
 //
 // The row's partial sums are reduced per j to [0, 2q) and added as packed
 // int16 pairs (below 8q for K <= 4: no carry between the halves); the
