@@ -697,17 +697,20 @@ __attribute__((amdgpu_waves_per_eu(TABLE ? 3 : 4))) void k_decrypt(const FusedIo
 
 }  // namespace
 
+// Grids of up to 4x the resident workgroups: the dispatcher then hands tiles
+// to whichever slot frees up, where a resident-sized grid left a tail of
+// workgroups on a last extra tile.  65,536 rank-3 instances: keygen 128 ->
+// 125 us, encryption 147 -> 144 us, matvec unchanged; 262,144: decryption
+// 243 -> 238-241 us.
+constexpr int kFusedOversub = 4;
+
 int launch_fused(int k, int mode, int s_bits, const FusedIo &io, const DeviceTables *tab, int ncu,
                  hipStream_t stream)
 {
   const size_t ntiles = (io.ninst + kTileInst - 1) / kTileInst;
-  // Up to 4x the resident workgroups: the dispatcher then hands tiles to
-  // whichever slot frees up, where a resident-sized grid left a tail of
-  // workgroups on a last extra tile (65,536 rank-3 instances: keygen
-  // 128 -> 125 us, encryption 147 -> 144 us, matvec unchanged).
-  constexpr int kOversub = 4;
   auto go = [&](auto kernel, int waves) {
-    const size_t cap = (size_t)ncu * resident_per_cu((const void *)kernel, 64 * waves) * kOversub;
+    const size_t cap =
+        (size_t)ncu * resident_per_cu((const void *)kernel, 64 * waves) * kFusedOversub;
     const dim3 grid((unsigned)(ntiles < cap ? ntiles : cap));
     hipLaunchKernelGGL(kernel, grid, dim3(64 * waves), 0, stream, io, tab);
     return 0;
@@ -750,7 +753,9 @@ int launch_fused(int k, int mode, int s_bits, const FusedIo &io, const DeviceTab
       if (s_bits)
         return -1;
       auto dec = [&](auto kernel) {
-        const size_t cap = (size_t)ncu * resident_per_cu((const void *)kernel, 64 * kDecryptWaves);
+        const size_t cap = (size_t)ncu *
+                           resident_per_cu((const void *)kernel, 64 * kDecryptWaves) *
+                           kFusedOversub;
         const size_t wgs = (ntiles + kDecryptWaves - 1) / kDecryptWaves;
         hipLaunchKernelGGL(kernel, dim3((unsigned)(wgs < cap ? wgs : cap)),
                            dim3(64 * kDecryptWaves), 0, stream, io, tab);
