@@ -85,6 +85,21 @@ def test_against_compiled_reference(oracle, reference, tk):
         assert oracle.montgomery_reduce(v) == reference.montgomery_reduce(v)
 
 
+def test_oracle_under_sanitizers():
+    """The restatement built with AddressSanitizer and UndefinedBehaviorSanitizer
+    (gcc, host only, single-threaded) runs every exported function over the
+    int16 range and checks its invariants (oracle/sanitize_check.c)."""
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None or shutil.which("make") is None:
+        pytest.skip("gcc or make not available")
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", "oracle")
+    r = subprocess.run(["make", "-s", "-C", here, "sanitize-check"], capture_output=True,
+                       text=True, timeout=600)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-4000:]
+    assert "sanitize-check ok" in r.stdout
+
+
 def test_reductions_whole_domain(oracle):
     """montgomery: all of [-q 2^15, q 2^15); barrett: every int16."""
     assert oracle.check_reductions() == 0
