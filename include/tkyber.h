@@ -290,8 +290,9 @@ int matvec_ntt_fused_packed(int16_t *t, const int16_t *A_hat, const uint8_t *s_p
 /* ---- autotuner ---------------------------------------------------------- */
 
 /* Times each MFMA tile shape x waves per workgroup x grid oversubscription
- * (1, 2, 4, 8 times the workgroups resident at once) on scratch data of
- * npoly polys (at most 2^20) on the current device and returns the fastest:
+ * (1, 2, 4, 8 times the workgroups resident at once) of the ragged-plan
+ * kernels (per-poly address table) on scratch data of npoly polys (at most
+ * 2^20) on the current device and returns the fastest:
  * method (TK_METHOD_MFMA16 or TK_METHOD_MFMA32), waves per workgroup (2 or 4)
  * and the oversubscription factor.  Results are cached per device, direction
  * and size octave for the life of the process.  Allocates and synchronises. */

@@ -59,8 +59,8 @@ int bucket_of(size_t npoly)
   return b;
 }
 
-int time_config(DevState *st, int shape, int waves, int wgs, bool inverse, int16_t *buf,
-                size_t npoly, hipStream_t s, float *ms_out)
+int time_config(DevState *st, int shape, int waves, int wgs, bool inverse, const RaggedTiles &io,
+                hipStream_t s, float *ms_out)
 {
   hipEvent_t e0, e1;
   if (hipEventCreate(&e0) != hipSuccess)
@@ -77,11 +77,11 @@ int time_config(DevState *st, int shape, int waves, int wgs, bool inverse, int16
   float best = 1e30f;
   int rc = TK_OK;
   for (int w = 0; w < 2; w++)
-    launch_mfma(shape, inverse, form, buf, buf, npoly, st->tables, ls, s);
+    launch_mfma_ragged(shape, inverse, form, io, st->tables, ls, s);
   for (int grp = 0; grp < kGroups && rc == TK_OK; grp++) {
     (void)hipEventRecord(e0, s);
     for (int i = 0; i < kPerGroup; i++)
-      launch_mfma(shape, inverse, form, buf, buf, npoly, st->tables, ls, s);
+      launch_mfma_ragged(shape, inverse, form, io, st->tables, ls, s);
     (void)hipEventRecord(e1, s);
     if ((rc = record_hip(hipEventSynchronize(e1))) != TK_OK)
       break;
@@ -107,13 +107,32 @@ int autotune(DevState *st, size_t npoly, int direction, TuneVal *out, hipStream_
       return TK_OK;
     }
   }
-  // Time on scratch data of the same size (capped at 2^20 polys); the
-  // kernels' cost does not depend on the values.
+  // Time the plans' own kernels (per-poly address table) on scratch data of
+  // the same size (capped at 2^20 polys): n contiguous polys, the table
+  // padded with a scratch poly as a plan's is.  The kernels' cost does not
+  // depend on the values.  Timing the linear-batch kernels instead ranked
+  // the shapes differently from the plans they were chosen for.
   const size_t n = npoly < ((size_t)1 << 20) ? npoly : ((size_t)1 << 20);
-  int16_t *buf = nullptr;
-  if (hipMalloc(&buf, n * kN * sizeof(int16_t)) != hipSuccess)
+  const size_t padded = (n + kRaggedPad - 1) / kRaggedPad * kRaggedPad;
+  const size_t b_data = (n + 1) * kN * sizeof(int16_t), b_ptrs = padded * sizeof(int16_t *);
+  void *mem = nullptr;
+  if (hipMalloc(&mem, b_data + b_ptrs) != hipSuccess)
     return TK_ENOMEM;
-  int rc = record_hip(hipMemsetAsync(buf, 0, n * kN * sizeof(int16_t), s));
+  int16_t *buf = static_cast<int16_t *>(mem);
+  int16_t **dptrs = reinterpret_cast<int16_t **>(static_cast<char *>(mem) + b_data);
+  std::vector<int16_t *> ptrs;
+  try {
+    ptrs.resize(padded);
+  } catch (...) {
+    (void)hipFree(mem);
+    return TK_ENOMEM;
+  }
+  for (size_t i = 0; i < padded; i++)
+    ptrs[i] = buf + (i < n ? i : n) * kN;
+  int rc = record_hip(hipMemsetAsync(buf, 0, b_data, s));
+  if (rc == TK_OK)
+    rc = record_hip(hipMemcpyAsync(dptrs, ptrs.data(), b_ptrs, hipMemcpyHostToDevice, s));
+  const RaggedTiles io{dptrs, n};
   // Candidates: tile shape x waves per workgroup x grid oversubscription
   // (grid = factor x workgroups resident at once).
   TuneVal best{TK_METHOD_MFMA16, 4, kDefaultOversub};
@@ -126,7 +145,7 @@ int autotune(DevState *st, size_t npoly, int direction, TuneVal *out, hipStream_
       for (int gi = 0; gi < 4 && rc == TK_OK; gi++) {
         const int shape = shapes[si], waves = wavess[wi], wgs = wgss[gi];
         float ms = 0;
-        rc = time_config(st, shape, waves, wgs, direction != 0, buf, n, s, &ms);
+        rc = time_config(st, shape, waves, wgs, direction != 0, io, s, &ms);
         if (rc == TK_OK && shape == 16 && waves == 4 && wgs == kDefaultOversub)
           default_ms = ms;
         if (rc == TK_OK && ms < best_ms) {
@@ -139,7 +158,7 @@ int autotune(DevState *st, size_t npoly, int direction, TuneVal *out, hipStream_
   if (default_ms <= best_ms * 1.01f)
     best = TuneVal{TK_METHOD_MFMA16, 4, kDefaultOversub};
   (void)hipStreamSynchronize(s);
-  (void)hipFree(buf);
+  (void)hipFree(mem);
   if (rc != TK_OK)
     return rc;
   std::lock_guard<std::mutex> lk(g_tune_mu);
