@@ -9,10 +9,9 @@
 namespace tk {
 
 constexpr int kStride = 264;  // int16 per poly slot in LDS (512 + 16 bytes)
-#ifndef TK_PASS_GROUP
-#define TK_PASS_GROUP 4
-#endif
-constexpr int kPassGroup = TK_PASS_GROUP;  // MFMA pairs in flight per pass step
+// MFMA pairs in flight per pass step: four columns or blocks per step keep 8
+// MFMAs in flight with 64-bit LDS accesses.
+constexpr int kPassGroup = 4;
 
 // ---------------------------------------------------------------------------
 // Tile staging: P polys, coalesced 16-byte global accesses, one LDS slab
