@@ -534,9 +534,14 @@ constexpr int kDecryptWaves = 4;
 // The row's partial sums are reduced per j to [0, 2q) and added as packed
 // int16 pairs (below 8q for K <= 4: no carry between the halves); the
 // inverse's first pass takes inputs below 8q, as the fused kernel's rows.
+//
+// Four waves per SIMD (at most 128 VGPRs; the LDS of four workgroups holds
+// the same 16 waves per CU).  The table-driven variant keeps a 64-bit
+// pointer per lane and access and runs at three rather than spill.
 template <int K, bool TABLE>
 __global__ __launch_bounds__(64 * kDecryptWaves)
-__attribute__((amdgpu_waves_per_eu(TABLE ? 3 : 4))) void k_decrypt(const FusedIo io, const DeviceTables *__restrict__ tab)
+__attribute__((amdgpu_waves_per_eu(TABLE ? 3 : 4))) void k_decrypt(
+    const FusedIo io, const DeviceTables *__restrict__ tab)
 {
   constexpr int P = kTileInst, W = kDecryptWaves;
   __shared__ const int16_t *s_ip[TABLE ? W : 1][TABLE ? P : 1][8];
