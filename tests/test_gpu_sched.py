@@ -128,3 +128,47 @@ def test_single_group_plan_matches_single_call(tk, gpu, oracle):
     torch.cuda.synchronize()
     assert torch.equal(u1, u2) and torch.equal(v1, v2)
     tk.lib().tk_fused_plan_destroy(plan)
+
+
+@pytest.mark.parametrize("mode,k,n", [("encrypt", 4, 100000), ("decrypt", 2, 240000)])
+def test_plan_groups_larger_than_the_grid(tk, gpu, mode, k, n):
+    """Groups of more tiles than the table-driven kernels' grid (4x their
+    resident workgroups), so that each workgroup loops over several tiles
+    with the pointer rows double-buffered by tile parity: the plan's outputs
+    against the contiguous kernels' (themselves pinned to the oracle), bit
+    for bit."""
+    import torch
+    g = torch.Generator(device=gpu)
+    g.manual_seed(77 + k)
+
+    def rnd(rows, lo=0, hi=3329):
+        return torch.randint(lo, hi, (rows, 256), dtype=torch.int16, device=gpu, generator=g)
+
+    print(f"plan group larger than the grid: {mode}, k = {k}, {n} instances", flush=True)
+    table = (tk.TkFusedRequest * 1)()
+    t = table[0]
+    t.k, t.ninst = k, n
+    if mode == "encrypt":
+        At, th, r = rnd(n * k * k), rnd(n * k), rnd(n * k, -3, 4)
+        e1, e2, m = rnd(n * k, -3, 4), rnd(n, -3, 4), rnd(n, 0, 2) * 1665
+        u1, v1 = tk.kpke_encrypt(At, th, r, e1, e2, m, k)
+        u2, v2 = torch.empty_like(u1), torch.empty_like(v1)
+        t.mode = tk.FUSED_ENCRYPT
+        t.out0, t.out1 = u2.data_ptr(), v2.data_ptr()
+        t.mat, t.mat2, t.x = At.data_ptr(), th.data_ptr(), r.data_ptr()
+        t.add0, t.add1, t.add2 = e1.data_ptr(), e2.data_ptr(), m.data_ptr()
+    else:
+        sh, u, v = rnd(n * k, -32768, 32768), rnd(n * k), rnd(n)
+        w1 = tk.kpke_decrypt(sh, u, v, k)
+        w2 = torch.empty_like(w1)
+        t.mode = tk.FUSED_DECRYPT
+        t.out0, t.mat, t.x, t.add0 = w2.data_ptr(), sh.data_ptr(), u.data_ptr(), v.data_ptr()
+    plan = ctypes.c_void_p()
+    assert tk.lib().tk_fused_plan_create(ctypes.byref(plan), ctypes.addressof(table), 1) == 0
+    assert tk.lib().tk_fused_plan_execute(plan, None) == 0
+    torch.cuda.synchronize()
+    if mode == "encrypt":
+        assert torch.equal(u1, u2) and torch.equal(v1, v2)
+    else:
+        assert torch.equal(w1, w2)
+    tk.lib().tk_fused_plan_destroy(plan)
