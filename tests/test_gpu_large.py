@@ -6,6 +6,8 @@
   (mod q) checked on every coefficient, chunk by chunk, on the device.
 * basemul over 2^23 pairs: every coefficient through the Montgomery identity,
   sampled rows against the oracle.
+* Rank-3 matvec and K-PKE batches larger than the fused kernels' grids:
+  sampled instances against the oracle.
 * Several host threads calling the library at once, each on its own stream
   and buffers, through the transforms, the fused product and a ragged plan.
 """
@@ -147,3 +149,55 @@ def test_concurrent_host_threads(tk, gpu, oracle):
         t.join(timeout=120)
     assert not any(t.is_alive() for t in threads), "a worker did not finish"
     assert not errors, errors
+
+
+@pytest.mark.parametrize("mode,n", [("matvec", 120000), ("keygen", 120000), ("encrypt", 100000),
+                                    ("decrypt", 300000)])
+def test_fused_batches_larger_than_the_grid(tk, gpu, oracle, mode, n):
+    """Rank-3 batches of more tiles than the fused kernels' grids (4x their
+    resident workgroups), so that each workgroup (each wave, for decryption)
+    loops over several tiles: sampled instances across the batch against
+    the oracle (raw int16 for the K-PKE cores, which end in poly_reduce;
+    centred for the matvec)."""
+    import torch
+    k = 3
+    g = torch.Generator(device=gpu)
+    g.manual_seed(91)
+
+    def rnd(*shape, lo=0, hi=Q):
+        return torch.randint(lo, hi, shape, dtype=torch.int16, device=gpu, generator=g)
+
+    rng = np.random.default_rng(13)
+    rows = np.unique(np.concatenate([[0, 1, 15, 16, n // 2, n - 17, n - 1],
+                                     rng.integers(0, n, 41)]))
+    ridx = torch.from_numpy(rows).to(gpu)
+
+    def pick(t, *shape):
+        return t.reshape(n, *shape).index_select(0, ridx).cpu().numpy()
+
+    print(f"fused batch larger than the grid: {mode}, {n} instances", flush=True)
+    if mode == "matvec":
+        A, s = rnd(n, k, k, N), rnd(n, k, N, lo=-2, hi=3)
+        got = pick(tk.matvec(A, s, k), k, N)
+        want = oracle.matvec(pick(A, k, k, N), pick(s, k, N), k).reshape(got.shape)
+        assert np.array_equal(got, centred(want))  # the oracle returns invntt's (-q, q) values
+    elif mode == "keygen":
+        A, s, e = rnd(n, k, k, N), rnd(n, k, N, lo=-2, hi=3), rnd(n, k, N, lo=-2, hi=3)
+        t_hat, s_hat = tk.kpke_keygen(A, s, e, k)
+        want_t, want_s = oracle.kpke_keygen(pick(A, k, k, N), pick(s, k, N), pick(e, k, N), k)
+        assert np.array_equal(pick(t_hat, k, N), want_t.reshape(-1, k, N))
+        assert np.array_equal(pick(s_hat, k, N), want_s.reshape(-1, k, N))
+    elif mode == "encrypt":
+        At, th, r = rnd(n, k, k, N), rnd(n, k, N), rnd(n, k, N, lo=-2, hi=3)
+        e1, e2, m = rnd(n, k, N, lo=-2, hi=3), rnd(n, N, lo=-2, hi=3), rnd(n, N, lo=0, hi=2) * 1665
+        u, v = tk.kpke_encrypt(At, th, r, e1, e2, m, k)
+        want_u, want_v = oracle.kpke_encrypt(pick(At, k, k, N), pick(th, k, N), pick(r, k, N),
+                                             pick(e1, k, N), pick(e2, N), pick(m, N), k)
+        assert np.array_equal(pick(u, k, N), want_u.reshape(-1, k, N))
+        assert np.array_equal(pick(v, N), want_v.reshape(-1, N))
+    else:
+        sh, u, v = rnd(n, k, N, lo=-32768, hi=32768), rnd(n, k, N), rnd(n, N)
+        w = tk.kpke_decrypt(sh, u, v, k)
+        want = oracle.kpke_decrypt(pick(sh, k, N), pick(u, k, N), pick(v, N), k)
+        assert np.array_equal(pick(w, N), want.reshape(-1, N))
+    torch.cuda.empty_cache()
