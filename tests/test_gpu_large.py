@@ -151,8 +151,8 @@ def test_concurrent_host_threads(tk, gpu, oracle):
     assert not errors, errors
 
 
-@pytest.mark.parametrize("mode,n", [("matvec", 120000), ("keygen", 120000), ("encrypt", 100000),
-                                    ("decrypt", 300000)])
+@pytest.mark.parametrize("mode,n", [("matvec", 120000), ("matvec_packed3", 120000),
+                                    ("keygen", 120000), ("encrypt", 100000), ("decrypt", 300000)])
 def test_fused_batches_larger_than_the_grid(tk, gpu, oracle, mode, n):
     """Rank-3 batches of more tiles than the fused kernels' grids (4x their
     resident workgroups), so that each workgroup (each wave, for decryption)
@@ -176,9 +176,12 @@ def test_fused_batches_larger_than_the_grid(tk, gpu, oracle, mode, n):
         return t.reshape(n, *shape).index_select(0, ridx).cpu().numpy()
 
     print(f"fused batch larger than the grid: {mode}, {n} instances", flush=True)
-    if mode == "matvec":
+    if mode.startswith("matvec"):
         A, s = rnd(n, k, k, N), rnd(n, k, N, lo=-2, hi=3)
-        got = pick(tk.matvec(A, s, k), k, N)
+        if mode == "matvec":
+            got = pick(tk.matvec(A, s, k), k, N)
+        else:
+            got = pick(tk.matvec_packed(A, tk.pack_small(s, 3), k, 3), k, N)
         want = oracle.matvec(pick(A, k, k, N), pick(s, k, N), k).reshape(got.shape)
         assert np.array_equal(got, centred(want))  # the oracle returns invntt's (-q, q) values
     elif mode == "keygen":
