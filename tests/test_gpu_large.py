@@ -6,7 +6,7 @@
   (mod q) checked on every coefficient, chunk by chunk, on the device.
 * basemul over 2^23 pairs: every coefficient through the Montgomery identity,
   sampled rows against the oracle.
-* Rank-3 matvec and K-PKE batches larger than the fused kernels' grids:
+* Matvec and K-PKE batches (ranks 2-4) larger than the fused kernels' grids:
   sampled instances against the oracle.
 * Several host threads calling the library at once, each on its own stream
   and buffers, through the transforms, the fused product and a ragged plan.
@@ -151,16 +151,19 @@ def test_concurrent_host_threads(tk, gpu, oracle):
     assert not errors, errors
 
 
-@pytest.mark.parametrize("mode,n", [("matvec", 120000), ("matvec_packed3", 120000),
-                                    ("keygen", 120000), ("encrypt", 100000), ("decrypt", 300000)])
-def test_fused_batches_larger_than_the_grid(tk, gpu, oracle, mode, n):
-    """Rank-3 batches of more tiles than the fused kernels' grids (4x their
-    resident workgroups), so that each workgroup (each wave, for decryption)
-    loops over several tiles: sampled instances across the batch against
-    the oracle (raw int16 for the K-PKE cores, which end in poly_reduce;
-    centred for the matvec)."""
+@pytest.mark.parametrize("mode,k,n", [("matvec", 3, 120000), ("matvec_packed3", 3, 120000),
+                                      ("keygen", 2, 120000), ("keygen", 3, 120000),
+                                      ("keygen", 4, 120000), ("encrypt", 2, 120000),
+                                      ("encrypt", 3, 100000), ("encrypt", 4, 120000),
+                                      ("decrypt", 2, 300000), ("decrypt", 3, 300000),
+                                      ("decrypt", 4, 300000)])
+def test_fused_batches_larger_than_the_grid(tk, gpu, oracle, mode, k, n):
+    """Batches of more tiles than the fused kernels' grids (4x their resident
+    workgroups), so that each workgroup (each wave, for decryption) loops
+    over several tiles: sampled instances across the batch against the
+    oracle (raw int16 for the K-PKE cores, which end in poly_reduce; centred
+    for the matvec)."""
     import torch
-    k = 3
     g = torch.Generator(device=gpu)
     g.manual_seed(91)
 
@@ -175,7 +178,7 @@ def test_fused_batches_larger_than_the_grid(tk, gpu, oracle, mode, n):
     def pick(t, *shape):
         return t.reshape(n, *shape).index_select(0, ridx).cpu().numpy()
 
-    print(f"fused batch larger than the grid: {mode}, {n} instances", flush=True)
+    print(f"fused batch larger than the grid: {mode}, k = {k}, {n} instances", flush=True)
     if mode.startswith("matvec"):
         A, s = rnd(n, k, k, N), rnd(n, k, N, lo=-2, hi=3)
         if mode == "matvec":
