@@ -182,6 +182,22 @@ def _as_polys(t, name="polys"):
     return t
 
 
+def _one_device(**tensors):
+    """All given tensors (None skipped) on one device: the C ABI takes raw
+    pointers and launches on the current device."""
+    devs = {name: t.device for name, t in tensors.items() if t is not None}
+    if len(set(devs.values())) > 1:
+        raise ValueError("tensors on different devices: " +
+                         ", ".join(f"{n} on {d}" for n, d in devs.items()))
+
+
+def _same_size(out, ref, name="out", ref_name="the input"):
+    """The C ABI takes the size from the input only, so an output that is
+    smaller would be written past its end on the device."""
+    if out.numel() != ref.numel():
+        raise ValueError(f"{name} holds {out.numel()} elements, {ref_name} {ref.numel()}")
+
+
 def _method(m):
     return METHODS[m] if isinstance(m, str) else int(m)
 
@@ -194,6 +210,8 @@ def ntt(polys, out=None, method="auto", form="default", stream=None):
     """Forward NTT of every poly (in place unless ``out`` is given)."""
     polys = _as_polys(polys)
     out = polys if out is None else _as_polys(out, "out")
+    _same_size(out, polys)
+    _one_device(polys=polys, out=out)
     _check(lib().ntt_batch_ex(out.data_ptr(), polys.data_ptr(), polys.numel() // N,
                               _method(method), _form(form), _stream_ptr(stream)), "ntt_batch")
     return out
@@ -203,6 +221,8 @@ def invntt(polys, out=None, method="auto", form="default", stream=None):
     """Inverse NTT (times 2^16, as invntt_tomont) of every poly."""
     polys = _as_polys(polys)
     out = polys if out is None else _as_polys(out, "out")
+    _same_size(out, polys)
+    _one_device(polys=polys, out=out)
     _check(lib().invntt_batch_ex(out.data_ptr(), polys.data_ptr(), polys.numel() // N,
                                  _method(method), _form(form), _stream_ptr(stream)), "invntt_batch")
     return out
@@ -215,6 +235,8 @@ def basemul(a, b, out=None, form="default", stream=None):
         raise ValueError("a and b differ in size")
     torch = _torch()
     out = torch.empty_like(a) if out is None else _as_polys(out, "out")
+    _same_size(out, a, ref_name="a")
+    _one_device(a=a, b=b, out=out)
     _check(lib().basemul_batch_ex(out.data_ptr(), a.data_ptr(), b.data_ptr(), a.numel() // N,
                                   _form(form), _stream_ptr(stream)), "basemul_batch")
     return out
@@ -230,6 +252,7 @@ def matvec(A_hat, s, k: int, out=None, stream=None):
     out = torch.empty_like(s) if out is None else _as_polys(out, "out")
     if s.numel() != ninst * k * N or out.numel() != s.numel():
         raise ValueError("s and out must hold ninst*k polys")
+    _one_device(A_hat=A_hat, s=s, out=out)
     _check(lib().matvec_ntt_fused(out.data_ptr(), A_hat.data_ptr(), s.data_ptr(), k, ninst,
                                   _stream_ptr(stream)), "matvec_ntt_fused")
     return out
@@ -264,6 +287,7 @@ def pack_small(polys, bits: int, out=None, stream=None):
     out, ppb = _as_packed(out, bits, "out")
     if out.numel() != npoly * ppb:
         raise ValueError("out has the wrong size")
+    _one_device(polys=polys, out=out)
     _check(lib().pack_small_batch(out.data_ptr(), polys.data_ptr(), npoly, bits,
                                   _stream_ptr(stream)), "pack_small_batch")
     return out
@@ -278,6 +302,7 @@ def ntt_packed(packed, bits: int, out=None, method="auto", form="default", strea
         else _as_polys(out, "out")
     if out.numel() != npoly * N:
         raise ValueError("out has the wrong size")
+    _one_device(packed=packed, out=out)
     _check(lib().ntt_batch_packed(out.data_ptr(), packed.data_ptr(), npoly, bits, _method(method),
                                   _form(form), _stream_ptr(stream)), "ntt_batch_packed")
     return out
@@ -295,6 +320,7 @@ def matvec_packed(A_hat, s_packed, k: int, bits: int, out=None, stream=None):
         else _as_polys(out, "out")
     if out.numel() != ninst * k * N:
         raise ValueError("out must hold ninst*k polys")
+    _one_device(A_hat=A_hat, s_packed=s_packed, out=out)
     _check(lib().matvec_ntt_fused_packed(out.data_ptr(), A_hat.data_ptr(), s_packed.data_ptr(), k,
                                          ninst, bits, _stream_ptr(stream)),
            "matvec_ntt_fused_packed")
@@ -362,6 +388,7 @@ def kpke_keygen(A_hat, s, e, k: int, t_hat=None, s_hat=None, stream=None):
     s_hat = torch.empty_like(s) if s_hat is None else _as_polys(s_hat, "s_hat")
     if t_hat.numel() != s.numel() or s_hat.numel() != s.numel():
         raise ValueError("t_hat and s_hat must hold ninst*k polys")
+    _one_device(A_hat=A_hat, s=s, e=e, t_hat=t_hat, s_hat=s_hat)
     _check(lib().kpke_keygen_core(t_hat.data_ptr(), s_hat.data_ptr(), A_hat.data_ptr(),
                                   s.data_ptr(), e.data_ptr(), k, ninst, _stream_ptr(stream)),
            "kpke_keygen_core")
@@ -384,6 +411,7 @@ def kpke_encrypt(At_hat, t_hat, r, e1, e2, m, k: int, u=None, v=None, stream=Non
         else _as_polys(v, "v")
     if u.numel() != ninst * k * N or v.numel() != ninst * N:
         raise ValueError("u must hold ninst*k polys and v ninst polys")
+    _one_device(At_hat=At_hat, t_hat=t_hat, r=r, e1=e1, e2=e2, m=m, u=u, v=v)
     _check(lib().kpke_encrypt_core(u.data_ptr(), v.data_ptr(), At_hat.data_ptr(), t_hat.data_ptr(),
                                    r.data_ptr(), e1.data_ptr(), e2.data_ptr(), _ptr_or_none(m), k,
                                    ninst, _stream_ptr(stream)), "kpke_encrypt_core")
@@ -401,6 +429,7 @@ def kpke_decrypt(s_hat, u, v, k: int, w=None, stream=None):
         else _as_polys(w, "w")
     if w.numel() != ninst * N:
         raise ValueError("w must hold ninst polys")
+    _one_device(s_hat=s_hat, u=u, v=v, w=w)
     _check(lib().kpke_decrypt_core(w.data_ptr(), s_hat.data_ptr(), u.data_ptr(), v.data_ptr(), k,
                                    ninst, _stream_ptr(stream)), "kpke_decrypt_core")
     return w

@@ -5,6 +5,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <map>
 #include <mutex>
 #include <utility>
@@ -266,6 +267,68 @@ int validate_fused(int k, int mode, const FusedIo &io)
     default:
       return TK_EINVAL;
   }
+}
+
+void fused_ranges(int k, int mode, const FusedIo &io, size_t req, std::vector<ReqRange> &outs,
+                  std::vector<ReqRange> &ins)
+{
+  const size_t pb = io.ninst * kN * sizeof(int16_t), vb = pb * k, mb = vb * k;
+  auto add = [&](std::vector<ReqRange> &v, const void *p, size_t bytes) {
+    if (p && bytes)
+      v.push_back(ReqRange{(uintptr_t)p, (uintptr_t)p + bytes, req});
+  };
+  switch (mode) {
+    case kFusedMatvec:
+      add(outs, io.out0, vb);
+      add(ins, io.mat, mb);
+      add(ins, io.x, vb);
+      break;
+    case kFusedKeygen:
+      add(outs, io.out0, vb);
+      add(outs, io.out1, vb);
+      add(ins, io.mat, mb);
+      add(ins, io.x, vb);
+      add(ins, io.add0, vb);
+      break;
+    case kFusedEncrypt:
+      add(outs, io.out0, vb);
+      add(outs, io.out1, pb);
+      add(ins, io.mat, mb);
+      add(ins, io.mat2, vb);
+      add(ins, io.x, vb);
+      add(ins, io.add0, vb);
+      add(ins, io.add1, pb);
+      add(ins, io.add2, pb);
+      break;
+    default:  // kFusedDecrypt
+      add(outs, io.out0, pb);
+      add(ins, io.mat, vb);
+      add(ins, io.x, vb);
+      add(ins, io.add0, pb);
+      break;
+  }
+}
+
+bool cross_request_overlap(std::vector<ReqRange> &outs, const std::vector<ReqRange> &ins)
+{
+  std::sort(outs.begin(), outs.end(),
+            [](const ReqRange &a, const ReqRange &b) { return a.lo < b.lo; });
+  // Outputs must be pairwise disjoint (within a request too: its own
+  // validation rejects overlapping outputs).
+  for (size_t i = 1; i < outs.size(); i++)
+    if (outs[i].lo < outs[i - 1].hi)
+      return true;
+  // Disjoint and sorted by start, so sorted by end as well: the outputs an
+  // input range meets are a contiguous run from the first that ends after
+  // its start.
+  for (const ReqRange &in : ins) {
+    auto it = std::upper_bound(outs.begin(), outs.end(), in.lo,
+                               [](uintptr_t v, const ReqRange &o) { return v < o.hi; });
+    for (; it != outs.end() && it->lo < in.hi; ++it)
+      if (it->req != in.req)
+        return true;
+  }
+  return false;
 }
 
 int run_fused_checked(int k, int mode, const FusedIo &io, void *stream)

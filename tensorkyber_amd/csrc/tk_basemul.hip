@@ -6,8 +6,10 @@
 // The grid covers the batch, one 16-byte chunk per lane: on MI355X that
 // streams at 6.2 TB/s, where a grid of the resident workgroups looping over
 // the batch reaches 4.5 TB/s (4.9-5.1 with 2-4 chunks per lane in flight).
-// The loop only takes over beyond kMaxGrid workgroups.
+// The loop only takes over beyond the largest grid (2^32 - 1 lanes: 2^27
+// polys).
 #include "tk_device.h"
+#include "tk_grid.h"
 #include "tk_launch.h"
 
 namespace tk {
@@ -66,12 +68,9 @@ int launch_basemul(int form, int16_t *r, const int16_t *a, const int16_t *b, siz
                    const DeviceTables *tab, int ncu, hipStream_t stream)
 {
   (void)ncu;
-  constexpr size_t kMaxGrid = (size_t)1 << 30;
-  const size_t nchunk = npoly * (kN / 8);
-  const size_t need = (nchunk + 255) / 256;
+  const unsigned grid = stream_grid(npoly * (kN / 8), 256);
   auto go = [&](auto kernel) {
-    const dim3 grid((unsigned)(need < kMaxGrid ? need : kMaxGrid));
-    hipLaunchKernelGGL(kernel, grid, dim3(256), 0, stream, r, a, b, npoly, tab);
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(256), 0, stream, r, a, b, npoly, tab);
     return 0;
   };
   switch (form) {

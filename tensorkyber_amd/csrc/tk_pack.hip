@@ -8,6 +8,7 @@
 // BITS = 2, 8 (16 bytes) for BITS = 3 or 4.  Dwords tile both buffers
 // contiguously (a poly is a whole number of them), so consecutive lanes read
 // and write consecutive addresses.
+#include "tk_grid.h"
 #include "tk_launch.h"
 #include "tk_mfma_core.h"
 
@@ -45,15 +46,14 @@ int launch_pack_small(int bits, uint8_t *out, const int16_t *in, size_t npoly, i
                       hipStream_t stream)
 {
   // The grid covers the batch (one dword per lane); the loop only takes
-  // over beyond 2^30 workgroups.  On MI355X this packs 2^20 polys into 2 bits
-  // in 99 us, against 116 us for a grid of 16 workgroups per CU looping over
-  // the batch (3 bits: 113-114 us either way).
+  // over beyond the largest grid (2^32 - 1 lanes: 2^28 polys at 2 bits).  On
+  // MI355X this packs 2^20 polys into 2 bits in 99 us, against 116 us for a
+  // grid of 16 workgroups per CU looping over the batch (3 bits: 113-114 us
+  // either way).
   (void)ncu;
   auto go = [&](auto kernel, int ppb) {
     const size_t ndw = npoly * (size_t)(ppb / 4);
-    const size_t want = (ndw + 255) / 256, cap = (size_t)1 << 30;
-    hipLaunchKernelGGL(kernel, dim3((unsigned)(want < cap ? want : cap)), dim3(256), 0, stream, out,
-                       in, ndw);
+    hipLaunchKernelGGL(kernel, dim3(stream_grid(ndw, 256)), dim3(256), 0, stream, out, in, ndw);
     return 0;
   };
   switch (bits) {

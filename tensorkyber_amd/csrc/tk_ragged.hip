@@ -237,6 +237,22 @@ int tk_ragged_plan_create(tk_ragged_plan **plan, const tk_request *table, size_t
   }
   if (total == 0)
     return TK_EINVAL;
+  // Every request is transformed in place by one kernel: a poly in two
+  // requests would be transformed twice, concurrently.
+  {
+    std::vector<ReqRange> outs;
+    try {
+      outs.reserve(nreq);
+    } catch (const std::bad_alloc &) {
+      return TK_ENOMEM;
+    }
+    for (size_t r = 0; r < nreq; r++)
+      if (table[r].npoly)
+        outs.push_back(ReqRange{(uintptr_t)table[r].base,
+                                (uintptr_t)table[r].base + (size_t)table[r].npoly * kN * 2, r});
+    if (cross_request_overlap(outs, {}))
+      return TK_EINVAL;
+  }
   DevState *st = nullptr;
   int rc = init_device(&st);
   if (rc)

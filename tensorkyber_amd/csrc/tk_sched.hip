@@ -168,21 +168,31 @@ int tk_fused_plan_create(tk_fused_plan **out, const tk_fused_request *reqs, size
   // (operation, rank).
   size_t count[4][5] = {};
   size_t total = 0;
-  for (size_t r = 0; r < nreq; r++) {
-    const tk_fused_request &q = reqs[r];
-    if (q.mode < TK_FUSED_MATVEC || q.mode > TK_FUSED_DECRYPT || q.k < 2 || q.k > 4)
-      return TK_EINVAL;
-    if (q.ninst == 0)
-      continue;
-    int rc = tk::validate_fused(q.k, q.mode, request_io(q));
-    if (rc)
-      return rc;
-    count[q.mode][q.k] += q.ninst;
-    total += q.ninst;
-    if (total > tk::kMaxPolys / 16)
-      return TK_EINVAL;
+  std::vector<tk::ReqRange> outs, ins;
+  try {
+    for (size_t r = 0; r < nreq; r++) {
+      const tk_fused_request &q = reqs[r];
+      if (q.mode < TK_FUSED_MATVEC || q.mode > TK_FUSED_DECRYPT || q.k < 2 || q.k > 4)
+        return TK_EINVAL;
+      if (q.ninst == 0)
+        continue;
+      int rc = tk::validate_fused(q.k, q.mode, request_io(q));
+      if (rc)
+        return rc;
+      count[q.mode][q.k] += q.ninst;
+      total += q.ninst;
+      if (total > tk::kMaxPolys / 16)
+        return TK_EINVAL;
+      tk::fused_ranges(q.k, q.mode, request_io(q), r, outs, ins);
+    }
+  } catch (const std::bad_alloc &) {
+    return TK_ENOMEM;
   }
   if (total == 0)
+    return TK_EINVAL;
+  // The groups run concurrently and one kernel serves many requests: an
+  // output shared with another request's buffers would be a data race.
+  if (tk::cross_request_overlap(outs, ins))
     return TK_EINVAL;
   tk::DevState *st = nullptr;
   int rc = tk::init_device(&st);

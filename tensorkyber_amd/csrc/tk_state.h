@@ -4,6 +4,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "tk_internal.h"
 #include "tk_launch.h"
 
@@ -35,6 +37,19 @@ int transform(bool inverse, int16_t *out, const int16_t *in, size_t npoly, int m
 // Argument rules of a fused call (k, required buffers, alignment, the one
 // exact alias each output may have, no other overlap); TK_OK or TK_EINVAL.
 int validate_fused(int k, int mode, const FusedIo &io);
+// Byte range [lo, hi) of a buffer of request `req` of a plan.
+struct ReqRange {
+  uintptr_t lo, hi;
+  size_t req;
+};
+// True if an output range overlaps any other output range, or an input range
+// of another request.  Ranges of one request are checked by that request's
+// own validation (an output may alias its own input exactly).  Sorts `outs`.
+bool cross_request_overlap(std::vector<ReqRange> &outs, const std::vector<ReqRange> &ins);
+// The output and input byte ranges of a fused call's buffers (non-null ones),
+// tagged with `req`.
+void fused_ranges(int k, int mode, const FusedIo &io, size_t req, std::vector<ReqRange> &outs,
+                  std::vector<ReqRange> &ins);
 // validate_fused, then one launch on stream.
 int run_fused_checked(int k, int mode, const FusedIo &io, void *stream);
 const HostMatrices &host_matrices();

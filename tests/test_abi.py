@@ -205,3 +205,61 @@ def test_canon_digest_is_representative_free(tk):
     x = tk.wl_uniform(4096, 9, -32768, 32768)
     y = (x.astype(np.int32) + Q * np.where(x > 0, -3, 3)).astype(np.int16)
     assert tk.canon_digest(x) == tk.canon_digest(y)
+
+
+def _created_or_nodev(tk, L, rc, plan, destroy):
+    """A table that passes validation reaches the device: ENODEV here, a plan
+    on a GPU (destroyed at once)."""
+    assert rc in (tk.ENODEV, tk.OK), rc
+    if rc == tk.OK:
+        assert destroy(plan) == tk.OK
+
+
+def test_ragged_plan_rejects_overlapping_requests(tk):
+    L = tk.lib()
+    buf = (ctypes.c_int16 * (N * 64 + 16))()
+    al = (ctypes.addressof(buf) + 15) & ~15
+    plan = ctypes.c_void_p()
+    reqs = (tk.TkRequest * 3)()
+    for r, (off, n) in enumerate(((0, 6), (6, 4), (20, 8))):
+        reqs[r].base, reqs[r].npoly, reqs[r].k, reqs[r].kind = al + off * 512, n, 2, 0
+
+    def create():
+        return L.tk_ragged_plan_create(ctypes.byref(plan), ctypes.addressof(reqs), 3, 0,
+                                       tk.METHOD_MFMA16, None)
+    _created_or_nodev(tk, L, create(), plan, L.tk_ragged_plan_destroy)  # touching, disjoint
+    reqs[1].base = al + 5 * 512                                        # one poly shared
+    assert create() == tk.EINVAL
+    reqs[1].base = al + 20 * 512                                       # same base as request 2
+    assert create() == tk.EINVAL
+    reqs[1].npoly = 0                                                  # empty: no range
+    _created_or_nodev(tk, L, create(), plan, L.tk_ragged_plan_destroy)
+
+
+def test_fused_plan_rejects_cross_request_overlap(tk):
+    L = tk.lib()
+    buf = (ctypes.c_int16 * (N * 128 + 16))()
+    al = (ctypes.addressof(buf) + 15) & ~15
+    P = 512
+    plan = ctypes.c_void_p()
+    reqs = (tk.TkFusedRequest * 2)()
+    # two rank-2 matvecs: A_hat 4 polys, s 2 polys, t 2 polys each
+    for r, base in enumerate((0, 16)):
+        reqs[r].mode, reqs[r].k, reqs[r].ninst = tk.FUSED_MATVEC, 2, 1
+        reqs[r].mat, reqs[r].x, reqs[r].out0 = (al + (base + 0) * P, al + (base + 4) * P,
+                                                al + (base + 6) * P)
+
+    def create():
+        return L.tk_fused_plan_create(ctypes.byref(plan), ctypes.addressof(reqs), 2)
+    _created_or_nodev(tk, L, create(), plan, L.tk_fused_plan_destroy)
+    reqs[1].mat = reqs[0].mat                              # a shared input is fine
+    _created_or_nodev(tk, L, create(), plan, L.tk_fused_plan_destroy)
+    reqs[1].x = reqs[0].out0                               # reads another request's output
+    assert create() == tk.EINVAL
+    reqs[1].x = al + 20 * P
+    reqs[1].out0 = al + 7 * P                              # outputs overlap by one poly
+    assert create() == tk.EINVAL
+    reqs[1].out0 = al + 1 * P                              # writes another request's input
+    assert create() == tk.EINVAL
+    reqs[1].out0 = reqs[1].x                               # its own input, exactly: allowed
+    _created_or_nodev(tk, L, create(), plan, L.tk_fused_plan_destroy)
