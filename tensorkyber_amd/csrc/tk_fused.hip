@@ -188,28 +188,27 @@ struct RoleTile;
 
 template <>
 struct RoleTile<false> {
-  Span sp;   // the tile's valid instances of the role
+  __amdgpu_buffer_rsrc_t r;
   int voff;  // lane's byte offset within chunk 0
   int sb;    // per-instance stride in bytes
   __device__ __forceinline__ RoleTile(const int16_t *base, size_t i0, int nvalid, int stride,
                                       int lane)
-      : sp{base ? base + i0 * (size_t)stride : nullptr,
-           base ? (uint32_t)nvalid * stride * 2 : 0u},
+      : r(span_rsrc(base ? base + i0 * (size_t)stride : nullptr,
+                    base ? (uint32_t)nvalid * stride * 2 : 0u)),
         voff((lane >> 5) * stride * 2 + (lane & 31) * 16),
         sb(stride * 2)
   {
   }
-  // The chunk's offset goes in the scalar operand, with the descriptor
-  // shrunk by it (Span::load); stores take the whole offset in the VGPR
-  // operand (see the note above span_rsrc).
   __device__ __forceinline__ v4i load(int it, int sub) const
   {
-    return sp.load(voff, 2 * it * sb + sub * 512);
+    return __builtin_bit_cast(
+        v4i, __builtin_amdgcn_raw_buffer_load_b128(r, voff, 2 * it * sb + sub * 512, kBufferNT));
   }
+  // The whole offset in the VGPR operand (see span_rsrc's note on stores).
   __device__ __forceinline__ void store(int it, int sub, v4i v) const
   {
     typedef unsigned int v4u __attribute__((ext_vector_type(4)));
-    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4u, v), span_rsrc(sp.base, sp.bytes),
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4u, v), r,
                                            voff + 2 * it * sb + sub * 512, 0, kBufferNT);
   }
 };
@@ -381,12 +380,14 @@ __global__ __launch_bounds__((64 * fused_waves<MODE, K>())) void k_fused(
         // instance t (packed_load_lanes' layout), through a descriptor over
         // the tile's valid instances.
         constexpr int PPB = packed_poly_bytes<SBITS>(), NDW = packed_lane_dwords<SBITS>();
-        const Span xs{static_cast<const uint8_t *>(io.x) + i0 * K * PPB,
-                      (uint32_t)nvalid * K * PPB};
+        const __amdgpu_buffer_rsrc_t xr =
+            span_rsrc(static_cast<const uint8_t *>(io.x) + i0 * K * PPB, (uint32_t)nvalid * K * PPB);
         uint32_t pv[NDW];
 #pragma unroll
         for (int i = 0; i < NDW / 4; i++) {
-          const v4i x = xs.load(tt * K * PPB + g * NDW * 4, w * PPB + i * 16);
+          const v4i x = __builtin_bit_cast(
+              v4i, __builtin_amdgcn_raw_buffer_load_b128(xr, tt * K * PPB + g * NDW * 4,
+                                                         w * PPB + i * 16, kBufferNT));
           pv[4 * i] = (uint32_t)x[0];
           pv[4 * i + 1] = (uint32_t)x[1];
           pv[4 * i + 2] = (uint32_t)x[2];

@@ -155,7 +155,7 @@ __device__ __forceinline__ void stage_in(const LinearTiles &io, size_t tile, int
                                          int16_t *const *, const StageRegs<P, LinearTiles> &,
                                          int lane)
 {
-  tile_in_span<P>(L, Span{io.in + tile * P * kN, tile_bytes<P>(tile, io.npoly, 2 * kN)}, lane);
+  tile_in_rsrc<P>(L, span_rsrc(io.in + tile * P * kN, tile_bytes<P>(tile, io.npoly, 2 * kN)), lane);
 }
 
 // Ragged tiles: every parked address is valid (padding points at the

@@ -24,18 +24,17 @@ constexpr int kPassGroup = 4;
 // tile needs no per-lane checks: the loads and the stores go out as one
 // unconditional batch each.
 //
-// The range check of a buffer access compares only its VGPR offset plus the
-// instruction's immediate with the descriptor's size; the scalar offset is
-// added after the check.  So:
-//   - loads keep the wave-uniform chunk offset in the (cheap, scalar)
-//     soffset operand, and each takes a descriptor whose size is the span's
-//     size minus that offset (Span::load): the check then covers the whole
-//     offset, and nothing past the span is read;
-//   - stores take their whole offset in the VGPR operand and a zero scalar
-//     offset.  That also keeps a hazard away: a 16-byte store whose data
-//     VGPRs are overwritten by the very next VALU instruction writes the new
-//     values on gfx950, and the compiler inserts the wait state that prevents
-//     this only for stores without a scalar offset register.
+// Stores take their whole offset in the VGPR operand and a zero scalar
+// offset.  A 16-byte store whose data VGPRs are overwritten by the very next
+// VALU instruction writes the new values on gfx950; the compiler inserts the
+// wait state that prevents this only for stores without a scalar offset
+// register (with one it assumes the hardware needs none).  Loads read no
+// data VGPRs and keep the cheaper scalar offset (the chunk offset): on
+// gfx950 the range check of these loads (raw buffer, VGPR offset enabled)
+// covers the scalar offset too.  Measured: loads of a full tile whose
+// descriptor size was reduced by their scalar offset returned zeros from
+// the second half of the tile, and batches ending at the last byte of their
+// allocation read nothing past it (tests/test_gpu_bounds.py).
 constexpr int kBufferRsrcWord3 = 0x00020000;  // gfx9 buffer descriptor: 32-bit data format
 constexpr int kBufferNT = 2;                  // cache policy: nontemporal
 
@@ -45,37 +44,19 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t span_rsrc(const void *base, ui
                                            kBufferRsrcWord3);
 }
 
-// bytes valid bytes from base (both wave-uniform).
-struct Span {
-  const void *base;
-  uint32_t bytes;
-  // 16 bytes at base + soff + voff, zero where soff + voff + 16 > bytes.
-  // soff must be wave-uniform; voff is the lane's own offset.
-  __device__ __forceinline__ v4i load(uint32_t voff, uint32_t soff) const
-  {
-    // max(bytes - soff, 0) on the scalar unit, written out: the compiler
-    // turns the C form into an unsigned saturating subtract, a VALU
-    // instruction, and a descriptor held in VGPRs then costs a readfirstlane
-    // loop per load.  Both operands are below 2^31.
-    int left;
-    asm("s_sub_i32 %0, %1, %2\n\ts_max_i32 %0, %0, 0" : "=&s"(left) : "s"(bytes), "s"(soff));
-    return __builtin_bit_cast(v4i, __builtin_amdgcn_raw_buffer_load_b128(
-                                       span_rsrc(base, (uint32_t)left), voff, soff, kBufferNT));
-  }
-};
-
 // Groups of eight 16-byte accesses per lane (one 16-poly tile, or half of a
 // 32-poly one), with a scheduling barrier between groups so that at most one
 // group's registers are live.
 template <int P>
-__device__ __forceinline__ void tile_in_span(int16_t *L, const Span &sp, int lane)
+__device__ __forceinline__ void tile_in_rsrc(int16_t *L, __amdgpu_buffer_rsrc_t r, int lane)
 {
 #pragma unroll
   for (int g0 = 0; g0 < P / 2; g0 += 8) {
     v4i v[8];
 #pragma unroll
     for (int i = 0; i < 8; i++)
-      v[i] = sp.load(lane * 16, (g0 + i) * 1024);
+      v[i] = __builtin_bit_cast(v4i, __builtin_amdgcn_raw_buffer_load_b128(
+                                         r, lane * 16, (g0 + i) * 1024, kBufferNT));
 #pragma unroll
     for (int i = 0; i < 8; i++) {
       const int q = (g0 + i) * 64 + lane, pp = q >> 5, ch = q & 31;

@@ -118,75 +118,3 @@ def test_scan_detects_the_pattern():
     assert stores == 1 and len(found) == 1
     safe = asm.replace("\tv_bfe_i32 v9", "\ts_nop 0\n\tv_bfe_i32 v9")
     assert _wide_store_hazards(safe)[0] == []
-
-
-_X4_LOAD = re.compile(r"^buffer_load_dwordx4\s+v\[\d+:\d+\],\s*v\d+,\s*s\[(\d+):(\d+)\],\s*(\S+)\s+offen")
-
-
-def _unbounded_staging_loads(asm):
-    """16-byte buffer loads whose scalar offset escapes the range check.
-
-    The range check of a buffer access covers only its VGPR offset and
-    immediate; a scalar offset is added after it.  A load with a non-zero
-    scalar offset sK is bounded only if its descriptor's size word (s[a+2])
-    was last written, possibly through register copies, as max(size - sK, 0):
-    ``s_sub_i32 sX, <size>, sK`` then ``s_max_i32 sX, sX, 0``."""
-    bad, loads = [], 0
-    for m in re.finditer(r"^(_Z\w+):", asm, re.M):
-        body = asm[m.end():asm.index(".Lfunc_end", m.end())]
-        ins = [l.strip() for l in body.split("\n")
-               if l.startswith("\t") and not l.strip().startswith((".", ";"))]
-        for j, line in enumerate(ins):
-            ld = _X4_LOAD.match(line)
-            if not ld:
-                continue
-            loads += 1
-            soff = ld.group(3)
-            if soff == "0":
-                continue
-            # the last write of the descriptor's size word before the load,
-            # followed back through register copies
-            reg, k = f"s{int(ld.group(1)) + 2}", j - 1
-            ok = False
-            while k >= 0:
-                w = re.match(r"^(s_\w+)\s+([\w]+),\s*(.*)$", ins[k])
-                if w and w.group(2) == reg:
-                    if w.group(1) == "s_mov_b32":
-                        reg = w.group(3).strip()
-                        k -= 1
-                        continue
-                    ok = (ins[k] == f"s_max_i32 {reg}, {reg}, 0" and k >= 1 and
-                          re.match(rf"^s_sub_i32 {reg}, \S+, {soff}$", ins[k - 1]) is not None)
-                    break
-                k -= 1
-            if not ok:
-                bad.append(f"{m.group(1)}: {line}")
-    return bad, loads
-
-
-def test_staging_loads_bounded_by_their_descriptor(compiled):
-    total = 0
-    for src, asm, _ in compiled:
-        bad, loads = _unbounded_staging_loads(asm)
-        total += loads
-        assert not bad, f"{src}: loads whose scalar offset escapes the range check:\n" + \
-            "\n".join(bad[:5])
-    assert total > 100, "no 16-byte buffer loads found: the scan is not reading the assembly"
-
-
-def test_bounded_load_scan_detects_the_pattern():
-    """The scan flags a chunk offset in soffset over an unshrunk descriptor."""
-    good = ("_Zfake:\n\ts_sub_i32 s2, s10, s27\n\ts_max_i32 s2, s2, 0\n"
-            "\tbuffer_load_dwordx4 v[92:95], v74, s[0:3], s27 offen nt\n"
-            "\tbuffer_load_dwordx4 v[96:99], v74, s[0:3], 0 offen nt\n\ts_endpgm\n.Lfunc_end0:\n")
-    assert _unbounded_staging_loads(good) == ([], 2)
-    bad = good.replace("\ts_sub_i32 s2, s10, s27\n\ts_max_i32 s2, s2, 0\n", "\ts_mov_b32 s2, s10\n")
-    assert len(_unbounded_staging_loads(bad)[0]) == 1
-    # the clamped size computed elsewhere and copied into the descriptor
-    copied = good.replace("\ts_sub_i32 s2, s10, s27\n\ts_max_i32 s2, s2, 0\n",
-                          "\ts_sub_i32 vcc_hi, s10, s27\n\ts_max_i32 vcc_hi, vcc_hi, 0\n"
-                          "\ts_mov_b32 s2, vcc_hi\n")
-    assert _unbounded_staging_loads(copied) == ([], 2)
-    # clamped by another load's offset
-    other = good.replace("s_sub_i32 s2, s10, s27", "s_sub_i32 s2, s10, s28")
-    assert len(_unbounded_staging_loads(other)[0]) == 1

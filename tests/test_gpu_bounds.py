@@ -2,14 +2,13 @@
 
 The MFMA and fused kernels stage a tile through a buffer descriptor whose
 size is the tile's valid bytes, so that a partial last tile needs no per-lane
-checks.  The range check of a buffer access covers only its VGPR offset; the
-chunk offset the staging loads pass in the scalar operand is covered by
-shrinking each load's descriptor by it.  Here every buffer of a call is
-placed so that the batch ends at the last byte of its own hipMalloc (or
+checks; the staging loads pass the chunk offset as the scalar offset, which
+the gfx950 range check covers for these loads.  Here every buffer of a call
+is placed so that the batch ends at the last byte of its own hipMalloc (or
 hipHostMalloc, for the kernels that read pinned host memory over PCIe), with
 batch sizes that leave the last tile partial, and the results are checked
-against the oracle on every coefficient.  Earlier tests always left guard
-polys or allocator slack after the batch.
+against the oracle on every coefficient.  The other tests leave guard polys
+or allocator slack after the batch.
 """
 import ctypes
 
