@@ -110,12 +110,15 @@ typedef struct {
   uint16_t kind;
 } tk_request;
 
-/* Host-built table of nreq requests; the plan copies the device address of
- * every requested poly to device memory once, picks the kernel shape with the autotuner (or takes
- * method MFMA16 / MFMA32 as given) and records the launch in a hipGraph.
+/* Host-built table of nreq requests; the plan cuts every request into tiles
+ * of the kernel's tile height (16 or 32 polys) and copies the table of tiles
+ * ({first poly, number of polys}, 16 bytes per tile) to device memory once,
+ * picks the kernel shape with the autotuner (or takes method MFMA16 / MFMA32
+ * as given) and records the launch in a hipGraph.
  * direction: 0 = forward (canonical out), 1 = inverse (centred out).
  * Transforms are in place.  Plan creation allocates and synchronises; the
- * execute and launch calls do neither.  Requests must not overlap. */
+ * execute and launch calls do neither.  Requests must not overlap one
+ * another (TK_EINVAL). */
 typedef struct tk_ragged_plan tk_ragged_plan;
 int tk_ragged_plan_create(tk_ragged_plan **plan, const tk_request *table, size_t nreq,
                           int direction, int method, void *stream);
@@ -289,13 +292,17 @@ int matvec_ntt_fused_packed(int16_t *t, const int16_t *A_hat, const uint8_t *s_p
 
 /* ---- autotuner ---------------------------------------------------------- */
 
-/* Times each MFMA tile shape x waves per workgroup x grid oversubscription
- * (1, 2, 4, 8 times the workgroups resident at once) of the ragged-plan
- * kernels (per-poly address table) on scratch data of npoly polys (at most
- * 2^20) on the current device and returns the fastest:
- * method (TK_METHOD_MFMA16 or TK_METHOD_MFMA32), waves per workgroup (2 or 4)
- * and the oversubscription factor.  Results are cached per device, direction
- * and size octave for the life of the process.  Allocates and synchronises. */
+/* Times each MFMA tile shape x waves per workgroup (2, 4) x grid
+ * oversubscription (4, 8, 12 times the workgroups resident at once) of the
+ * ragged-plan kernel on scratch data of npoly polys (at most 2^20) laid out
+ * as one request, on the current device, in interleaved rounds, and returns
+ * the fastest -- unless it beats the linear batches' shape (MFMA16, 4 waves,
+ * 8x) by less than 2%, which is then kept: method (TK_METHOD_MFMA16 or
+ * TK_METHOD_MFMA32), waves per workgroup and the oversubscription factor.
+ * Plans with method TK_METHOD_AUTO run the same search on their own
+ * requests.  Results are cached per device, direction, size octave and
+ * request-size octave for the life of the process.  Allocates and
+ * synchronises. */
 int tk_autotune(size_t npoly, int direction, int *best_method, int *best_waves_per_wg,
                 int *best_oversub, void *stream);
 

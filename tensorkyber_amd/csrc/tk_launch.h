@@ -16,17 +16,22 @@ struct LinearTiles {
   size_t npoly;
 };
 
-// A ragged batch of requests, transformed in place: the requests' polys are
-// numbered in table order, and ptrs[g] is the device address of poly g.  One
-// table read per poly (8 of every 1,032 bytes moved) replaces a dependent
-// walk of the request table in front of every tile.  The table holds npoly
-// rounded up to a multiple of kRaggedPad entries; those past npoly point at
-// a scratch poly of the plan, so the last tile loads and stores like the
-// others instead of branching per poly.
-constexpr int kRaggedPad = 32;  // the largest tile, in polys
+// A ragged batch of requests, transformed in place, as a table of tiles:
+// each request is cut into tiles of the kernel's tile height P (16 or 32
+// polys), the last one partial, and tiles[t] holds tile t's first poly and
+// its number of polys (1..P).  A wave reads its tile's entry with one scalar
+// load, issued a tile ahead, and stages the tile through a buffer descriptor
+// over its polys as a linear batch does.  16 bytes per tile (1 byte per poly
+// at P = 16) where a table of per-poly addresses took 8 per poly.
+struct TileRef {
+  int16_t *base;
+  uint32_t npoly;
+  uint32_t reserved;
+};
+static_assert(sizeof(TileRef) == 16, "one 16-byte scalar load per tile");
 struct RaggedTiles {
-  int16_t *const *ptrs;
-  size_t npoly;
+  const TileRef *tiles;
+  size_t ntiles;
 };
 
 // A contiguous batch of small-coefficient polys in the packed BITS-bit form
@@ -49,8 +54,7 @@ struct PackedTiles {
 // at 8; the inverse slows again from 16).  Packed batches use 2: their waves load few bytes per tile, and
 // on MI355X 2^20 packed polys take 137-140 us (2 bits) and 142-145 us
 // (3 bits) at 1 or 2, against 146 and 153 us at 8 and 168 and 164 us at 32.
-// Ragged plans start from 4 and let the autotuner choose.
-constexpr int kDefaultOversub = 4;
+// Ragged plans let the autotuner choose, starting from the linear shape.
 constexpr int kLinearOversub = 8;
 constexpr int kLinearOversub32 = 12;
 constexpr int kPackedOversub = 2;
@@ -69,6 +73,7 @@ int launch_butterfly(bool inverse, int form, int16_t *out, const int16_t *in, si
 int launch_mfma(int shape, bool inverse, int form, int16_t *out, const int16_t *in, size_t npoly,
                 const DeviceTables *tab, const LaunchShape &ls, hipStream_t stream);
 
+// io.tiles must have been cut for shape's tile height (16 or 32 polys).
 int launch_mfma_ragged(int shape, bool inverse, int form, const RaggedTiles &io,
                        const DeviceTables *tab, const LaunchShape &ls, hipStream_t stream);
 

@@ -36,68 +36,57 @@ __device__ __forceinline__ int32_t bias_adjust()
   return FORM == kFormCentred ? 0 : -kCentre;
 }
 
-// Per-tile poly addresses.  Linear (and packed) tiles need none: they are
-// staged through a buffer descriptor over the tile (stage_in, stage_out
-// below).  Ragged tiles take them from the plan's address
-// table: lanes 0..P-1 read one address each one tile ahead (tile_fetch), so
-// the read is in flight while the current tile is staged, and park them in
-// LDS, where the staging loops read them back; that keeps P addresses out of
-// vector registers across the two passes.
+// Per-tile inputs and outputs, wave-uniform: where tile `tile`'s polys are
+// read and written and how many of its P polys there are.  Linear and packed
+// tiles follow from the tile index; ragged tiles come from the plan's tile
+// table, one scalar load per tile (the table is read-only for the kernel's
+// life, so it is read through the constant address space).
+struct TileIo {
+  const int16_t *in;
+  int16_t *out;
+  uint32_t npoly;
+};
+
 template <int P>
-__device__ __forceinline__ int16_t *tile_fetch(const LinearTiles &, size_t, int)
+__device__ __forceinline__ size_t tile_count(const LinearTiles &io)
 {
-  return nullptr;
+  return (io.npoly + P - 1) / P;
 }
 
 template <int P>
-__device__ __forceinline__ void tile_prepare(const LinearTiles &, size_t, int, int16_t *,
-                                             int16_t **)
+__device__ __forceinline__ TileIo tile_io(const LinearTiles &io, size_t tile)
 {
+  const size_t left = io.npoly - tile * P;
+  return TileIo{io.in + tile * P * kN, io.out + tile * P * kN,
+                (uint32_t)(left < (size_t)P ? left : (size_t)P)};
 }
 
 template <int P>
-__device__ __forceinline__ int16_t *tile_fetch(const RaggedTiles &io, size_t tile, int lane)
+__device__ __forceinline__ size_t tile_count(const RaggedTiles &io)
 {
-  static_assert(kRaggedPad % P == 0, "the padded table covers every tile");
-  const uint64_t g = (uint64_t)tile * P + lane;
-  return lane < P && (uint64_t)tile * P < io.npoly ? io.ptrs[g] : nullptr;
+  return io.ntiles;
 }
 
 template <int P>
-__device__ __forceinline__ void tile_prepare(const RaggedTiles &, size_t, int lane, int16_t *pre,
-                                             int16_t **lds)
+__device__ __forceinline__ TileIo tile_io(const RaggedTiles &io, size_t tile)
 {
-  if (lane < P)
-    lds[lane] = pre;
-  wave_sync();
-}
-
-template <int P>
-__device__ __forceinline__ const int16_t *tile_src(const RaggedTiles &, size_t, int pp,
-                                                   int16_t *const *lds)
-{
-  return lds[pp];
-}
-
-template <int P>
-__device__ __forceinline__ int16_t *tile_dst(const RaggedTiles &, size_t, int pp,
-                                             int16_t *const *lds)
-{
-  return lds[pp];
-}
-
-// Packed tiles: output addresses as for linear tiles; the input is staged
-// from the packed buffer (stage_in below).
-template <int P, int BITS>
-__device__ __forceinline__ int16_t *tile_fetch(const PackedTiles<BITS> &, size_t, int)
-{
-  return nullptr;
+  typedef const __attribute__((address_space(4))) TileRef *ConstTiles;
+  const ConstTiles t = (ConstTiles)io.tiles + tile;
+  int16_t *const base = t->base;
+  return TileIo{base, base, t->npoly};
 }
 
 template <int P, int BITS>
-__device__ __forceinline__ void tile_prepare(const PackedTiles<BITS> &, size_t, int, int16_t *,
-                                             int16_t **)
+__device__ __forceinline__ size_t tile_count(const PackedTiles<BITS> &io)
 {
+  return (io.npoly + P - 1) / P;
+}
+
+template <int P, int BITS>
+__device__ __forceinline__ TileIo tile_io(const PackedTiles<BITS> &io, size_t tile)
+{
+  const size_t left = io.npoly - tile * P;
+  return TileIo{nullptr, io.out + tile * P * kN, (uint32_t)(left < (size_t)P ? left : (size_t)P)};
 }
 
 // Staging registers: packed tiles are small (4-16 dwords per lane), so the
@@ -139,86 +128,29 @@ __device__ __forceinline__ void stage_load(const PackedTiles<BITS> &io, size_t t
   }
 }
 
-// Bytes of the polys of tile `tile` that lie in a batch of npoly.
-template <int P>
-__device__ __forceinline__ uint32_t tile_bytes(size_t tile, size_t npoly, uint32_t poly_bytes)
+// Stage the tile into the slab L: int16 tiles (linear and ragged) through a
+// buffer descriptor over the tile's polys, packed polys from the staging
+// registers.
+template <int P, class Tiles>
+__device__ __forceinline__ void stage_in(const Tiles &, const TileIo &t, int16_t *L,
+                                         const StageRegs<P, Tiles> &, int lane)
 {
-  const size_t left = npoly - tile * P;
-  return (uint32_t)(left < (size_t)P ? left : (size_t)P) * poly_bytes;
-}
-
-// Stage tile `tile` into the slab L: linear tiles through a buffer
-// descriptor over the tile, ragged polys through their parked addresses,
-// packed polys from the staging registers.
-template <int P>
-__device__ __forceinline__ void stage_in(const LinearTiles &io, size_t tile, int16_t *L,
-                                         int16_t *const *, const StageRegs<P, LinearTiles> &,
-                                         int lane)
-{
-  tile_in_rsrc<P>(L, span_rsrc(io.in + tile * P * kN, tile_bytes<P>(tile, io.npoly, 2 * kN)), lane);
-}
-
-// Ragged tiles: every parked address is valid (padding points at the
-// plan's scratch poly), so the loads and stores are unconditional.
-template <int P>
-__device__ __forceinline__ void stage_in(const RaggedTiles &io, size_t tile, int16_t *L,
-                                         int16_t *const *lds, const StageRegs<P, RaggedTiles> &,
-                                         int lane)
-{
-  v4i v[P / 2];
-#pragma unroll
-  for (int it = 0; it < P / 2; it++) {
-    const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
-    v[it] = ld_stream(reinterpret_cast<const v4i *>(tile_src<P>(io, tile, pp, lds)) + ch);
-  }
-#pragma unroll
-  for (int it = 0; it < P / 2; it++) {
-    const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
-    *reinterpret_cast<v4i *>(L + pp * kStride + ch * 8) = v[it];
-  }
+  tile_in_rsrc<P>(L, span_rsrc(t.in, t.npoly * 2 * kN), lane);
 }
 
 template <int P, int BITS>
-__device__ __forceinline__ void stage_in(const PackedTiles<BITS> &, size_t, int16_t *L,
-                                         int16_t *const *, const StageRegs<P, PackedTiles<BITS>> &r,
-                                         int lane)
+__device__ __forceinline__ void stage_in(const PackedTiles<BITS> &, const TileIo &, int16_t *L,
+                                         const StageRegs<P, PackedTiles<BITS>> &r, int lane)
 {
   if constexpr (P != 16)
     packed_park<P, BITS>(L, r.v, lane);
 }
 
-// Write the slab of tile `tile` to its output polys.
+// Write the slab to the tile's output polys.
 template <int P>
-__device__ __forceinline__ void stage_out(const LinearTiles &io, size_t tile, const int16_t *L,
-                                          int16_t *const *, int lane)
+__device__ __forceinline__ void stage_out(const TileIo &t, const int16_t *L, int lane)
 {
-  tile_out_rsrc<P>(L, span_rsrc(io.out + tile * P * kN, tile_bytes<P>(tile, io.npoly, 2 * kN)),
-                   lane);
-}
-
-template <int P, int BITS>
-__device__ __forceinline__ void stage_out(const PackedTiles<BITS> &io, size_t tile,
-                                          const int16_t *L, int16_t *const *, int lane)
-{
-  tile_out_rsrc<P>(L, span_rsrc(io.out + tile * P * kN, tile_bytes<P>(tile, io.npoly, 2 * kN)),
-                   lane);
-}
-
-template <int P>
-__device__ __forceinline__ void stage_out(const RaggedTiles &io, size_t tile, const int16_t *L,
-                                          int16_t *const *lds, int lane)
-{
-  v4i v[P / 2];
-#pragma unroll
-  for (int it = 0; it < P / 2; it++) {
-    const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
-    v[it] = *reinterpret_cast<const v4i *>(L + pp * kStride + ch * 8);
-  }
-#pragma unroll
-  for (int it = 0; it < P / 2; it++) {
-    const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
-    st_stream(reinterpret_cast<v4i *>(tile_dst<P>(io, tile, pp, lds)) + ch, v[it]);
-  }
+  tile_out_rsrc<P>(L, span_rsrc(t.out, t.npoly * 2 * kN), lane);
 }
 
 // Bits per coefficient of packed tiles, 0 for int16 tiles.
@@ -238,7 +170,6 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(3)))
 {
   constexpr int P = 16;
   __shared__ __align__(16) int16_t s_poly[WAVES][P * kStride];
-  __shared__ int16_t *s_ptr[WAVES][2 * P];
   __shared__ __align__(16) int32_t s_bias[16][16];
   const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), g = lane >> 4, t = lane & 15;
 
@@ -271,16 +202,17 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(3)))
   // than are resident, which also balances load across CUs.  The passes are
   // hidden: the kernel runs at the speed of its staging alone.  A register
   // prefetch of the next tile (a wave of occupancy) measured no faster.
-  const size_t ntiles = (io.npoly + P - 1) / P;
+  // The next tile's description (for ragged tiles a scalar load of the
+  // table) is issued at the top of each iteration, a tile ahead of its use.
+  const size_t ntiles = tile_count<P>(io);
   const size_t stride = (size_t)gridDim.x * WAVES;
   size_t tile = (size_t)blockIdx.x * WAVES + w;
-  int16_t *pre = tile_fetch<P>(io, tile, lane);
+  TileIo cur = tile_io<P>(io, tile < ntiles ? tile : 0);
   StageRegs<P, Tiles> sr;
   stage_load<P>(io, tile, sr, lane);
   for (; tile < ntiles; tile += stride) {
-    tile_prepare<P>(io, tile, lane, pre, s_ptr[w]);
-    pre = tile_fetch<P>(io, tile + stride, lane);
-    stage_in<P>(io, tile, L, s_ptr[w], sr, lane);
+    const TileIo nxt = tile_io<P>(io, tile + stride < ntiles ? tile + stride : tile);
+    stage_in<P>(io, cur, L, sr, lane);
     wave_sync();
     if (!INV) {
       if constexpr (PB != 0)
@@ -298,8 +230,9 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(3)))
     }
     wave_sync();
     stage_load<P>(io, tile + stride, sr, lane);
-    stage_out<P>(io, tile, L, s_ptr[w], lane);
+    stage_out<P>(cur, L, lane);
     wave_sync();
+    cur = nxt;
   }
 }
 
@@ -310,7 +243,6 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2)))
 {
   constexpr int P = 32;
   __shared__ __align__(16) int16_t s_poly[WAVES][P * kStride];
-  __shared__ int16_t *s_ptr[WAVES][2 * P];
   __shared__ __align__(16) int32_t s_bias[16][16];
   const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), h = lane >> 5, t = lane & 31;
 
@@ -336,16 +268,15 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2)))
   int16_t *L = s_poly[w];
   int16_t *Pp = L + t * kStride;
   // 32 polys per wave: no register budget left for a prefetch buffer.
-  const size_t ntiles = (io.npoly + P - 1) / P;
+  const size_t ntiles = tile_count<P>(io);
   const size_t stride = (size_t)gridDim.x * WAVES;
   size_t tile = (size_t)blockIdx.x * WAVES + w;
-  int16_t *pre = tile_fetch<P>(io, tile, lane);
+  TileIo cur = tile_io<P>(io, tile < ntiles ? tile : 0);
   StageRegs<P, Tiles> sr;
   stage_load<P>(io, tile, sr, lane);
   for (; tile < ntiles; tile += stride) {
-    tile_prepare<P>(io, tile, lane, pre, s_ptr[w]);
-    pre = tile_fetch<P>(io, tile + stride, lane);
-    stage_in<P>(io, tile, L, s_ptr[w], sr, lane);
+    const TileIo nxt = tile_io<P>(io, tile + stride < ntiles ? tile + stride : tile);
+    stage_in<P>(io, cur, L, sr, lane);
     wave_sync();
     if (!INV) {
       pass_m32<kFormInter>(Pp, h, a1, bias1a, bias1b);
@@ -358,9 +289,18 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2)))
     }
     wave_sync();
     stage_load<P>(io, tile + stride, sr, lane);
-    stage_out<P>(io, tile, L, s_ptr[w], lane);
+    stage_out<P>(cur, L, lane);
     wave_sync();
+    cur = nxt;
   }
+}
+
+inline size_t host_tile_count(const LinearTiles &io, int P) { return (io.npoly + P - 1) / P; }
+inline size_t host_tile_count(const RaggedTiles &io, int) { return io.ntiles; }
+template <int BITS>
+inline size_t host_tile_count(const PackedTiles<BITS> &io, int P)
+{
+  return (io.npoly + P - 1) / P;
 }
 
 template <class K, class Tiles>
@@ -369,7 +309,7 @@ int launch_persistent(K kernel, int waves, int P, int ncu, int oversub, const Ti
 {
   const int block = 64 * waves;
   const int per_cu = resident_per_cu((const void *)kernel, block) * (oversub > 0 ? oversub : 1);
-  const size_t ntiles = (io.npoly + P - 1) / P;
+  const size_t ntiles = host_tile_count(io, P);
   const size_t need = (ntiles + waves - 1) / waves;
   const size_t cap = (size_t)ncu * per_cu;
   const dim3 grid((unsigned)(need < cap ? need : cap));

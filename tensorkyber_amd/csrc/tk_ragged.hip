@@ -1,17 +1,21 @@
 // Ragged batches (many uneven ML-KEM-512/768/1024 vectors and matrices in
 // one launch) and the launch-shape autotuner.
 //
-// A plan flattens the request table into a global poly numbering, uploads
-// the device address of every poly once, picks the MFMA tile shape / waves
-// per workgroup / grid with the autotuner, and records the persistent kernel
-// into a hipGraph.  Executing the plan replays
-// the graph: one launch whatever the number of requests, no host work.
+// A plan cuts every request into tiles of the kernel's tile height (16 or
+// 32 polys, the last tile of a request partial) and uploads the table of
+// tiles once: 16 bytes per tile, {first poly, number of polys}.  The
+// persistent MFMA kernel walks that table (one scalar load per tile, issued
+// a tile ahead) and stages each tile through a buffer descriptor over its
+// polys exactly as it stages a linear batch.  The autotuner picks the tile
+// shape, waves per workgroup and grid from timings of the plan's own
+// requests, and the launch is recorded into a hipGraph: executing the plan
+// replays it, one launch whatever the number of requests, no host work.
 #include <hip/hip_runtime.h>
 #include <string.h>
 
 #include <map>
-#include <new>
 #include <mutex>
+#include <new>
 #include <tuple>
 #include <vector>
 
@@ -39,10 +43,11 @@ namespace tk {
 namespace {
 
 struct TuneKey {
-  int device, direction, bucket;
+  int device, direction, bucket, req_bucket;
   bool operator<(const TuneKey &o) const
   {
-    return std::tie(device, direction, bucket) < std::tie(o.device, o.direction, o.bucket);
+    return std::tie(device, direction, bucket, req_bucket) <
+           std::tie(o.device, o.direction, o.bucket, o.req_bucket);
   }
 };
 struct TuneVal {
@@ -51,54 +56,80 @@ struct TuneVal {
 std::mutex g_tune_mu;
 std::map<TuneKey, TuneVal> g_tune;
 
-int bucket_of(size_t npoly)
+int octave(size_t n)
 {
   int b = 0;
-  while (b < 40 && ((size_t)1 << (b + 1)) <= npoly)
+  while (b < 40 && ((size_t)1 << (b + 1)) <= n)
     b++;
   return b;
 }
 
-int time_config(DevState *st, int shape, int waves, int wgs, bool inverse, const RaggedTiles &io,
-                hipStream_t s, float *ms_out)
+// The shape linear batches use: MFMA16 tiles, 4 waves per workgroup, a grid
+// of 8x the resident workgroups.  Another shape must beat it clearly.
+constexpr TuneVal kBaseline{TK_METHOD_MFMA16, 4, kLinearOversub};
+constexpr float kSwitchMargin = 0.98f;  // switch only for >= 2% less time
+
+// Cuts requests (base, npoly) into tiles of P polys.
+void cut_tiles(const tk_request *table, size_t nreq, int P, std::vector<TileRef> &out)
 {
-  hipEvent_t e0, e1;
-  if (hipEventCreate(&e0) != hipSuccess)
-    return TK_EDEVICE;
-  if (hipEventCreate(&e1) != hipSuccess) {
-    (void)hipEventDestroy(e0);
-    return TK_EDEVICE;
-  }
+  out.clear();
+  for (size_t r = 0; r < nreq; r++)
+    for (uint32_t i = 0; i < table[r].npoly; i += (uint32_t)P) {
+      const uint32_t n = table[r].npoly - i;
+      out.push_back(TileRef{table[r].base + (size_t)i * kN, n < (uint32_t)P ? n : (uint32_t)P, 0});
+    }
+}
+
+size_t count_tiles(const tk_request *table, size_t nreq, int P)
+{
+  size_t n = 0;
+  for (size_t r = 0; r < nreq; r++)
+    n += (table[r].npoly + (size_t)P - 1) / (size_t)P;
+  return n;
+}
+
+struct Candidate {
+  int shape, waves, oversub;
+  float ms;
+};
+
+// One group of kPerGroup launches back to back on s: per-launch time.
+int time_group(DevState *st, const Candidate &c, bool inverse, const RaggedTiles &io,
+               hipStream_t s, hipEvent_t e0, hipEvent_t e1, float *ms)
+{
+  constexpr int kPerGroup = 4;
   const int form = inverse ? kFormCentred : kFormCanon;
-  const LaunchShape ls{st->ncu, waves, wgs};
-  // Two warm-up launches, then the best of three groups of four launches
-  // back to back (per-launch time), which rides out clock and queue noise.
-  constexpr int kGroups = 3, kPerGroup = 4;
-  float best = 1e30f;
-  int rc = TK_OK;
-  for (int w = 0; w < 2; w++)
-    launch_mfma_ragged(shape, inverse, form, io, st->tables, ls, s);
-  for (int grp = 0; grp < kGroups && rc == TK_OK; grp++) {
-    (void)hipEventRecord(e0, s);
-    for (int i = 0; i < kPerGroup; i++)
-      launch_mfma_ragged(shape, inverse, form, io, st->tables, ls, s);
-    (void)hipEventRecord(e1, s);
-    if ((rc = record_hip(hipEventSynchronize(e1))) != TK_OK)
-      break;
-    float ms = 0;
-    (void)hipEventElapsedTime(&ms, e0, e1);
-    if (ms / kPerGroup < best)
-      best = ms / kPerGroup;
-  }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  *ms_out = best;
+  const LaunchShape ls{st->ncu, c.waves, c.oversub};
+  (void)hipEventRecord(e0, s);
+  for (int i = 0; i < kPerGroup; i++)
+    launch_mfma_ragged(c.shape, inverse, form, io, st->tables, ls, s);
+  (void)hipEventRecord(e1, s);
+  int rc = record_hip(hipEventSynchronize(e1));
+  if (rc == TK_OK)
+    rc = after_launch();
+  float t = 0;
+  (void)hipEventElapsedTime(&t, e0, e1);
+  *ms = t / kPerGroup;
   return rc;
 }
 
-int autotune(DevState *st, size_t npoly, int direction, TuneVal *out, hipStream_t s)
+// Times every candidate on requests laid out as the plan's are and returns
+// the fastest, or the baseline unless another beats it by the margin.
+//
+// The requests are rebased onto a scratch pool: with their own gaps when the
+// pool spans at most 1.25x their bytes (the kernels' cost does not depend on
+// the values), else back to back.  At most 2^20 polys of requests are timed
+// (a prefix of the table).  The candidates are timed in interleaved rounds,
+// each keeping its best group, so a clock ramp or a noisy neighbour affects
+// them alike.
+int autotune_requests(DevState *st, const tk_request *table, size_t nreq, int direction,
+                      TuneVal *out, hipStream_t s)
 {
-  const TuneKey key{st->device, direction, bucket_of(npoly)};
+  constexpr size_t kCap = (size_t)1 << 20;
+  size_t total = 0, used = 0;
+  for (size_t r = 0; r < nreq; r++)
+    total += table[r].npoly;
+  const TuneKey key{st->device, direction, octave(total), octave(total / nreq)};
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     auto it = g_tune.find(key);
@@ -107,63 +138,109 @@ int autotune(DevState *st, size_t npoly, int direction, TuneVal *out, hipStream_
       return TK_OK;
     }
   }
-  // Time the plans' own kernels (per-poly address table) on scratch data of
-  // the same size (capped at 2^20 polys): n contiguous polys, the table
-  // padded with a scratch poly as a plan's is.  The kernels' cost does not
-  // depend on the values.  Timing the linear-batch kernels instead ranked
-  // the shapes differently from the plans they were chosen for.
-  const size_t n = npoly < ((size_t)1 << 20) ? npoly : ((size_t)1 << 20);
-  const size_t padded = (n + kRaggedPad - 1) / kRaggedPad * kRaggedPad;
-  const size_t b_data = (n + 1) * kN * sizeof(int16_t), b_ptrs = padded * sizeof(int16_t *);
-  void *mem = nullptr;
-  if (hipMalloc(&mem, b_data + b_ptrs) != hipSuccess)
-    return TK_ENOMEM;
-  int16_t *buf = static_cast<int16_t *>(mem);
-  int16_t **dptrs = reinterpret_cast<int16_t **>(static_cast<char *>(mem) + b_data);
-  std::vector<int16_t *> ptrs;
+  std::vector<tk_request> reqs;
+  std::vector<TileRef> t16, t32;
   try {
-    ptrs.resize(padded);
-  } catch (...) {
-    (void)hipFree(mem);
+    size_t taken = 0;
+    for (size_t r = 0; r < nreq && taken < kCap; r++) {
+      if (!table[r].npoly)
+        continue;
+      tk_request q = table[r];
+      if (q.npoly > kCap - taken)
+        q.npoly = (uint32_t)(kCap - taken);
+      taken += q.npoly;
+      reqs.push_back(q);
+    }
+    used = taken;
+  } catch (const std::bad_alloc &) {
     return TK_ENOMEM;
   }
-  for (size_t i = 0; i < padded; i++)
-    ptrs[i] = buf + (i < n ? i : n) * kN;
-  int rc = record_hip(hipMemsetAsync(buf, 0, b_data, s));
+  uintptr_t lo = UINTPTR_MAX, hi = 0;
+  for (const tk_request &q : reqs) {
+    lo = (uintptr_t)q.base < lo ? (uintptr_t)q.base : lo;
+    const uintptr_t e = (uintptr_t)q.base + (size_t)q.npoly * kN * 2;
+    hi = e > hi ? e : hi;
+  }
+  const size_t bytes = used * kN * 2;
+  const bool keep_gaps = hi - lo <= bytes + bytes / 4;
+  const size_t pool_bytes = keep_gaps ? hi - lo : bytes;
+  void *pool = nullptr;
+  if (hipMalloc(&pool, pool_bytes) != hipSuccess)
+    return TK_ENOMEM;
+  size_t at = 0;
+  for (tk_request &q : reqs) {
+    const size_t off = keep_gaps ? (uintptr_t)q.base - lo : at;
+    q.base = reinterpret_cast<int16_t *>(static_cast<char *>(pool) + off);
+    at += (size_t)q.npoly * kN * 2;
+  }
+  void *dt = nullptr;
+  int rc = TK_OK;
+  try {
+    cut_tiles(reqs.data(), reqs.size(), 16, t16);
+    cut_tiles(reqs.data(), reqs.size(), 32, t32);
+  } catch (const std::bad_alloc &) {
+    rc = TK_ENOMEM;
+  }
+  const size_t b16 = t16.size() * sizeof(TileRef), b32 = t32.size() * sizeof(TileRef);
+  if (rc == TK_OK && hipMalloc(&dt, b16 + b32) != hipSuccess)
+    rc = TK_ENOMEM;
   if (rc == TK_OK)
-    rc = record_hip(hipMemcpyAsync(dptrs, ptrs.data(), b_ptrs, hipMemcpyHostToDevice, s));
-  const RaggedTiles io{dptrs, n};
-  // Candidates: tile shape x waves per workgroup x grid oversubscription
-  // (grid = factor x workgroups resident at once).
-  TuneVal best{TK_METHOD_MFMA16, 4, kDefaultOversub};
-  float best_ms = 1e30f, default_ms = 1e30f;
-  const int shapes[2] = {16, 32};
-  const int wavess[2] = {2, 4};
-  const int wgss[4] = {1, 2, 4, 8};
-  for (int si = 0; si < 2 && rc == TK_OK; si++)
-    for (int wi = 0; wi < 2 && rc == TK_OK; wi++)
-      for (int gi = 0; gi < 4 && rc == TK_OK; gi++) {
-        const int shape = shapes[si], waves = wavess[wi], wgs = wgss[gi];
-        float ms = 0;
-        rc = time_config(st, shape, waves, wgs, direction != 0, io, s, &ms);
-        if (rc == TK_OK && shape == 16 && waves == 4 && wgs == kDefaultOversub)
-          default_ms = ms;
-        if (rc == TK_OK && ms < best_ms) {
-          best_ms = ms;
-          best = TuneVal{shape == 32 ? TK_METHOD_MFMA32 : TK_METHOD_MFMA16, waves, wgs};
-        }
-      }
-  // Keep the default shape unless another is clearly faster: differences
-  // inside the timing noise would otherwise make the choice vary by run.
-  if (default_ms <= best_ms * 1.01f)
-    best = TuneVal{TK_METHOD_MFMA16, 4, kDefaultOversub};
+    rc = record_hip(hipMemsetAsync(pool, 0, pool_bytes, s));
+  if (rc == TK_OK)
+    rc = record_hip(hipMemcpyAsync(dt, t16.data(), b16, hipMemcpyHostToDevice, s));
+  if (rc == TK_OK)
+    rc = record_hip(hipMemcpyAsync(static_cast<char *>(dt) + b16, t32.data(), b32,
+                                   hipMemcpyHostToDevice, s));
+  const RaggedTiles io16{static_cast<const TileRef *>(dt), t16.size()};
+  const RaggedTiles io32{reinterpret_cast<const TileRef *>(static_cast<char *>(dt) + b16),
+                         t32.size()};
+
+  std::vector<Candidate> cand;
+  for (int shape : {16, 32})
+    for (int waves : {2, 4})
+      for (int oversub : {4, 8, 12})
+        cand.push_back(Candidate{shape, waves, oversub, 1e30f});
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (rc == TK_OK)
+    rc = record_hip(hipEventCreate(&e0));
+  if (rc == TK_OK)
+    rc = record_hip(hipEventCreate(&e1));
+  const bool inverse = direction != 0;
+  // Warm-up (clocks, code objects), then interleaved rounds.
+  constexpr int kRounds = 4;
+  for (int round = -1; round < kRounds && rc == TK_OK; round++)
+    for (Candidate &c : cand) {
+      float ms = 0;
+      rc = time_group(st, c, inverse, c.shape == 32 ? io32 : io16, s, e0, e1, &ms);
+      if (rc != TK_OK)
+        break;
+      if (round >= 0 && ms < c.ms)
+        c.ms = ms;
+    }
+  if (e0)
+    (void)hipEventDestroy(e0);
+  if (e1)
+    (void)hipEventDestroy(e1);
   (void)hipStreamSynchronize(s);
-  (void)hipFree(mem);
+  if (dt)
+    (void)hipFree(dt);
+  (void)hipFree(pool);
   if (rc != TK_OK)
     return rc;
+  const Candidate *best = &cand[0], *base = nullptr;
+  for (const Candidate &c : cand) {
+    if (c.ms < best->ms)
+      best = &c;
+    if (c.shape == 16 && c.waves == kBaseline.waves && c.oversub == kBaseline.oversub)
+      base = &c;
+  }
+  TuneVal v = kBaseline;
+  if (base && best != base && best->ms < base->ms * kSwitchMargin)
+    v = TuneVal{best->shape == 32 ? TK_METHOD_MFMA32 : TK_METHOD_MFMA16, best->waves,
+                best->oversub};
   std::lock_guard<std::mutex> lk(g_tune_mu);
-  g_tune[key] = best;
-  *out = best;
+  g_tune[key] = v;
+  *out = v;
   return TK_OK;
 }
 
@@ -203,8 +280,11 @@ int tk_autotune(size_t npoly, int direction, int *best_method, int *best_waves_p
   int rc = init_device(&st);
   if (rc)
     return rc;
+  // One contiguous request of npoly polys (at most 2^20 are timed).
+  const size_t n = npoly < ((size_t)1 << 20) ? npoly : ((size_t)1 << 20);
+  tk_request one{reinterpret_cast<int16_t *>((uintptr_t)256), (uint32_t)n, 2, 0};
   TuneVal v{};
-  if ((rc = autotune(st, npoly, direction, &v, (hipStream_t)stream)))
+  if ((rc = autotune_requests(st, &one, 1, direction, &v, (hipStream_t)stream)))
     return rc;
   if (best_method)
     *best_method = v.method;
@@ -259,35 +339,27 @@ int tk_ragged_plan_create(tk_ragged_plan **plan, const tk_request *table, size_t
     return rc;
   hipStream_t s = (hipStream_t)stream;
 
-  TuneVal tv{TK_METHOD_MFMA16, 4, kDefaultOversub};
+  TuneVal tv = kBaseline;
   if (method == TK_METHOD_AUTO) {
-    if ((rc = autotune(st, total, direction, &tv, s)))
+    if ((rc = autotune_requests(st, table, nreq, direction, &tv, s)))
       return rc;
-  } else {
-    tv.method = method == TK_METHOD_MFMA32 ? TK_METHOD_MFMA32 : TK_METHOD_MFMA16;
-    tv.waves = 4;
-    tv.oversub = kDefaultOversub;
+  } else if (method == TK_METHOD_MFMA32) {
+    tv = TuneVal{TK_METHOD_MFMA32, 4, kLinearOversub32};
   }
   const int shape = tv.method == TK_METHOD_MFMA32 ? 32 : 16;
 
-  // Host image of the address table: poly g of the flattened numbering,
-  // padded to a multiple of kRaggedPad entries that point at the plan's
-  // scratch poly (after the table in the same allocation).
-  const size_t padded = (total + kRaggedPad - 1) / kRaggedPad * kRaggedPad;
-  const size_t b_ptrs = padded * sizeof(int16_t *);
-  const size_t b_alloc = b_ptrs + kN * sizeof(int16_t);
-  std::vector<int16_t *> ptrs;
+  std::vector<TileRef> tiles;
   try {
-    ptrs.resize(padded);
+    tiles.reserve(count_tiles(table, nreq, shape));
+    cut_tiles(table, nreq, shape, tiles);
   } catch (const std::bad_alloc &) {
     return TK_ENOMEM;
   }
-  size_t g = 0;
-  for (size_t r = 0; r < nreq; r++)
-    for (uint32_t i = 0; i < table[r].npoly; i++)
-      ptrs[g++] = table[r].base + (size_t)i * kN;
+  const size_t b_tiles = tiles.size() * sizeof(TileRef);
 
-  tk_ragged_plan *p = new tk_ragged_plan();
+  tk_ragged_plan *p = new (std::nothrow) tk_ragged_plan();
+  if (!p)
+    return TK_ENOMEM;
   p->device = st->device;
   p->direction = direction;
   p->shape = shape;
@@ -297,18 +369,15 @@ int tk_ragged_plan_create(tk_ragged_plan **plan, const tk_request *table, size_t
   p->npoly = total;
   p->nreq = nreq;
   p->tab = st->tables;
-  if (hipMalloc(&p->dmem, b_alloc) != hipSuccess) {
+  if (hipMalloc(&p->dmem, b_tiles) != hipSuccess) {
     plan_free(p);
     return TK_ENOMEM;
   }
-  int16_t *scratch = reinterpret_cast<int16_t *>(static_cast<char *>(p->dmem) + b_ptrs);
-  for (size_t i = total; i < padded; i++)
-    ptrs[i] = scratch;
-  if ((rc = record_hip(hipMemcpy(p->dmem, ptrs.data(), b_ptrs, hipMemcpyHostToDevice)))) {
+  if ((rc = record_hip(hipMemcpy(p->dmem, tiles.data(), b_tiles, hipMemcpyHostToDevice)))) {
     plan_free(p);
     return rc;
   }
-  p->tiles = RaggedTiles{(int16_t *const *)p->dmem, total};
+  p->tiles = RaggedTiles{static_cast<const TileRef *>(p->dmem), tiles.size()};
 
   // Record the launch into a graph on a private capture stream.
   hipStream_t cap = nullptr;

@@ -195,3 +195,34 @@ def test_pinned_host_batch_ending_at_allocation_end(tk, hip, oracle, method):
             assert np.array_equal(hout.reshape(npoly, N), centred(oracle.invntt(fwd)))
     finally:
         L.tk_host_pipeline_destroy(p)
+
+
+@pytest.mark.parametrize("method", ["mfma16", "mfma32"])
+def test_ragged_requests_ending_at_allocation_end(tk, hip, oracle, method):
+    """A ragged plan whose requests each end at the last byte of their own
+    allocation, with sizes that leave every request's last tile partial."""
+    L = tk.lib()
+    sizes = (1, 17, 33, 47)
+    reqs = (tk.TkRequest * len(sizes))()
+    xs, bases = [], []
+    for r, n in enumerate(sizes):
+        x = _uniform(tk, n, 940 + n)
+        b = hip.at_end(n * PB)
+        hip.put(b, x)
+        reqs[r].base, reqs[r].npoly, reqs[r].k, reqs[r].kind = b, n, 2, 0
+        xs.append(x)
+        bases.append(b)
+    for direction in (0, 1):
+        print(f"ragged {method} direction {direction}", flush=True)
+        plan = ctypes.c_void_p()
+        assert L.tk_ragged_plan_create(ctypes.byref(plan), ctypes.addressof(reqs), len(sizes),
+                                       direction, tk.METHODS[method], None) == 0
+        assert L.tk_ragged_plan_execute(plan, None) == 0
+        hip.sync()
+        for x, b in zip(xs, bases):
+            got = hip.get(b, x.shape)
+            want = canon(oracle.ntt(x)) if direction == 0 else centred(oracle.invntt(x))
+            assert np.array_equal(got, want), f"{method} dir {direction} request of {len(x)}"
+            hip.put(b, got)
+        xs = [hip.get(b, x.shape) for x, b in zip(xs, bases)]
+        assert L.tk_ragged_plan_destroy(plan) == 0
