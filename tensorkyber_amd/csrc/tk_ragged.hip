@@ -11,6 +11,8 @@
 // requests, and the launch is recorded into a hipGraph: executing the plan
 // replays it, one launch whatever the number of requests, no host work.
 #include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <map>
@@ -234,6 +236,10 @@ int autotune_requests(DevState *st, const tk_request *table, size_t nreq, int di
     if (c.shape == 16 && c.waves == kBaseline.waves && c.oversub == kBaseline.oversub)
       base = &c;
   }
+  if (getenv("TK_DEBUG"))
+    for (const Candidate &c : cand)
+      fprintf(stderr, "tkyber autotune dir %d: shape %d waves %d oversub %2d: %.4f ms\n",
+              direction, c.shape, c.waves, c.oversub, c.ms);
   TuneVal v = kBaseline;
   if (base && best != base && best->ms < base->ms * kSwitchMargin)
     v = TuneVal{best->shape == 32 ? TK_METHOD_MFMA32 : TK_METHOD_MFMA16, best->waves,
