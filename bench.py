@@ -179,54 +179,116 @@ def cpu_baseline(x):
                       f"{t1:.2f} s"}
 
 
+def hip_graph_runtime():
+    """hipGraphInstantiate / Launch / AddChildGraphNode of the HIP runtime this
+    process already loaded (torch's, shared with libtkyber), via ctypes."""
+    import ctypes
+    path = None
+    with open("/proc/self/maps") as f:
+        for line in f:
+            if "libamdhip64.so" in line:
+                path = line.split()[-1]
+                break
+    lib = ctypes.CDLL(path)
+    vp, sz, pvp = ctypes.c_void_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)
+    for name, args in (("hipGraphCreate", [pvp, ctypes.c_uint]), ("hipGraphDestroy", [vp]),
+                       ("hipGraphAddChildGraphNode", [pvp, vp, vp, sz, vp]),
+                       ("hipGraphInstantiate", [pvp, vp, vp, vp, sz]),
+                       ("hipGraphLaunch", [vp, vp]), ("hipGraphExecDestroy", [vp])):
+        getattr(lib, name).argtypes, getattr(lib, name).restype = args, ctypes.c_int
+    return lib
+
+
 def bench_ragged(tk, torch, steps):
     """Configuration 4: 2^20 polys in ragged ML-KEM requests (seed 4), forward
-    and inverse plans (autotuned persistent kernel, hipGraph replay)."""
+    and inverse plans (autotuned persistent kernel over a per-tile table).
+
+    * ``rt_ms``: tk_ragged_plan_execute of the forward then the inverse plan
+      (one dispatch each), `steps` round trips between two events;
+    * ``fwd_ms`` / ``inv_ms``: each plan alone, per launch;
+    * ``rt_ms_graph_replay``: each plan's recorded graph (tk_ragged_plan_graph)
+      instantiated and launched, two graph launches per round trip;
+    * ``rt_ms_one_graph``: both plans as child nodes of one graph, one launch
+      per round trip (the cost of the graph launch paid once per batch)."""
     import ctypes
     N = 256
     reqs, pool = tk.wl_ragged_layout(1 << 20, 4)
     x = tk.wl_uniform(pool * N, 4).reshape(pool, N)
     d = torch.from_numpy(x).to("cuda:0")
     L = tk.lib()
-    plans = []
+    plans, create_s = [], []
     for direction in (0, 1):
         table = (tk.TkRequest * len(reqs))()
         for r, (off, n, k, kind) in enumerate(reqs):
             table[r].base = d.data_ptr() + int(off) * N * 2
             table[r].npoly, table[r].k, table[r].kind = int(n), int(k), int(kind)
         p = ctypes.c_void_p()
+        t0 = time.perf_counter()
         rc = L.tk_ragged_plan_create(ctypes.byref(p), ctypes.addressof(table), len(reqs),
                                      direction, tk.METHOD_AUTO, None)
+        create_s.append(time.perf_counter() - t0)
         if rc != 0:
             raise SystemExit(f"tk_ragged_plan_create failed: {rc}")
         plans.append(p)
-    info = [ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_size_t()]
-    L.tk_ragged_plan_info(plans[0], *[ctypes.byref(v) for v in info])
+    shapes = []
+    for p in plans:
+        info = [ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_size_t()]
+        L.tk_ragged_plan_info(p, *[ctypes.byref(v) for v in info])
+        shapes.append({3: "mfma16", 4: "mfma32"}.get(info[0].value, info[0].value) +
+                      f" at {info[2].value}x")
     stream = torch.cuda.current_stream()
     sp = ctypes.c_void_p(stream.cuda_stream)
 
-    def timed(run):
-        warm(torch, lambda: (run(plans[0], sp), run(plans[1], sp)))
+    def timed(fn):
+        warm(torch, fn)
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record(stream)
         for _ in range(steps):
-            run(plans[0], sp)
-            run(plans[1], sp)
+            fn()
         e1.record(stream)
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / steps
 
-    ms = timed(L.tk_ragged_plan_execute)
-    ms_direct = timed(L.tk_ragged_plan_launch)
+    hip = hip_graph_runtime()
+    graphs = []
+    for p in plans:
+        g = ctypes.c_void_p()
+        L.tk_ragged_plan_graph(p, ctypes.byref(g))
+        graphs.append(g.value)
+
+    def instantiate(g):
+        ex = ctypes.c_void_p()
+        assert hip.hipGraphInstantiate(ctypes.byref(ex), g, None, None, 0) == 0
+        return ex.value
+    execs = [instantiate(g) for g in graphs]
+    parent = ctypes.c_void_p()
+    hip.hipGraphCreate(ctypes.byref(parent), 0)
+    n0, n1 = ctypes.c_void_p(), ctypes.c_void_p()
+    hip.hipGraphAddChildGraphNode(ctypes.byref(n0), parent, None, 0, graphs[0])
+    dep = (ctypes.c_void_p * 1)(n0.value)
+    hip.hipGraphAddChildGraphNode(ctypes.byref(n1), parent, dep, 1, graphs[1])
+    both = instantiate(parent.value)
+
+    ms = timed(lambda: (L.tk_ragged_plan_execute(plans[0], sp),
+                        L.tk_ragged_plan_execute(plans[1], sp)))
+    fwd = timed(lambda: L.tk_ragged_plan_execute(plans[0], sp))
+    inv = timed(lambda: L.tk_ragged_plan_execute(plans[1], sp))
+    ms_graphs = timed(lambda: (hip.hipGraphLaunch(execs[0], sp), hip.hipGraphLaunch(execs[1], sp)))
+    ms_one = timed(lambda: hip.hipGraphLaunch(both, sp))
+    for ex in execs + [both]:
+        hip.hipGraphExecDestroy(ex)
+    hip.hipGraphDestroy(parent.value)
     for p in plans:
         L.tk_ragged_plan_destroy(p)
     npoly = int(reqs[:, 1].sum())
     return {"workload": f"{len(reqs)} requests of 1..4096 polys (rank-k vectors, 25% k x k "
                         f"matrices), {npoly} polys in a {pool}-poly pool with gaps, seed 4",
-            "rt_poly_s": npoly / (ms / 1e3), "rt_ms": ms,
-            "rt_ms_direct_launch": ms_direct,
-            "method": {3: "mfma16", 4: "mfma32"}.get(info[0].value, info[0].value),
-            "oversub": info[2].value, "launch": "hipGraph replay of one persistent kernel"}
+            "rt_poly_s": npoly / (ms / 1e3), "rt_ms": ms, "fwd_ms": fwd, "inv_ms": inv,
+            "rt_fraction_of_roofline": npoly * 2 * BYTES_PER_POLY_PASS / (ms / 1e3) / 1e9 /
+            HBM_PEAK_GBS,
+            "rt_ms_graph_replay": ms_graphs, "rt_ms_one_graph": ms_one,
+            "plan_create_s": create_s, "shape_fwd": shapes[0], "shape_inv": shapes[1],
+            "launch": "tk_ragged_plan_execute: one dispatch of the persistent kernel per plan"}
 
 
 def bench_basemul(tk, torch, steps, npoly=1 << 20):

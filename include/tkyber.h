@@ -114,7 +114,7 @@ typedef struct {
  * of the kernel's tile height (16 or 32 polys) and copies the table of tiles
  * ({first poly, number of polys}, 16 bytes per tile) to device memory once,
  * picks the kernel shape with the autotuner (or takes method MFMA16 / MFMA32
- * as given) and records the launch in a hipGraph.
+ * as given) and records the launch of the persistent kernel in a hipGraph.
  * direction: 0 = forward (canonical out), 1 = inverse (centred out).
  * Transforms are in place.  Plan creation allocates and synchronises; the
  * execute and launch calls do neither.  Requests must not overlap one
@@ -122,10 +122,17 @@ typedef struct {
 typedef struct tk_ragged_plan tk_ragged_plan;
 int tk_ragged_plan_create(tk_ragged_plan **plan, const tk_request *table, size_t nreq,
                           int direction, int method, void *stream);
-/* Replays the captured graph on stream (NULL = the legacy default stream). */
+/* Enqueues the plan on stream (NULL = the legacy default stream): one
+ * dispatch of the persistent kernel, whatever the number of requests.  (A
+ * replay of the plan's one-node graph costs 4-10 us more per launch on
+ * ROCm 7; the graph is for composition, below.) */
 int tk_ragged_plan_execute(tk_ragged_plan *plan, void *stream);
-/* Launches the persistent kernel directly, without the graph. */
+/* The same as tk_ragged_plan_execute. */
 int tk_ragged_plan_launch(tk_ragged_plan *plan, void *stream);
+/* The plan's recorded launch as a hipGraph_t (owned by the plan, valid until
+ * it is destroyed), for callers that build a graph of their own work and add
+ * plans to it as child graph nodes (hipGraphAddChildGraphNode copies it). */
+int tk_ragged_plan_graph(tk_ragged_plan *plan, void **graph);
 /* Shape the plan runs: method (MFMA16 / MFMA32), polys per wave tile, grid
  * oversubscription (grid = factor x workgroups resident at once), and the
  * total number of polys. */

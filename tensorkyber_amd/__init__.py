@@ -122,6 +122,7 @@ def _declare(lib):
         "tk_ragged_plan_info": ([vp, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(i),
                                  ctypes.POINTER(sz)], i),
         "tk_ragged_plan_destroy": ([vp], i),
+        "tk_ragged_plan_graph": ([vp, ctypes.POINTER(vp)], i),
         "matvec_ntt_fused": ([vp, vp, vp, i, sz, vp], i),
         "tk_autotune": ([sz, i, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(i), vp], i),
     }

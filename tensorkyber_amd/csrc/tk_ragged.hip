@@ -8,13 +8,15 @@
 // a tile ahead) and stages each tile through a buffer descriptor over its
 // polys exactly as it stages a linear batch.  The autotuner picks the tile
 // shape, waves per workgroup and grid from timings of the plan's own
-// requests, and the launch is recorded into a hipGraph: executing the plan
-// replays it, one launch whatever the number of requests, no host work.
+// requests.  Executing the plan is one dispatch of that kernel whatever the
+// number of requests, with no host work; the launch is also recorded into a
+// hipGraph, which callers can embed in their own graphs.
 #include <hip/hip_runtime.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <map>
 #include <mutex>
 #include <new>
@@ -37,8 +39,7 @@ struct tk_ragged_plan {
   void *dmem = nullptr;
   tk::RaggedTiles tiles{};
   const tk::DeviceTables *tab = nullptr;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;  // the recorded launch, for composition
 };
 
 namespace tk {
@@ -92,16 +93,21 @@ size_t count_tiles(const tk_request *table, size_t nreq, int P)
 
 struct Candidate {
   int shape, waves, oversub;
-  float ms;
+  float ms;                // median over the rounds
+  std::vector<float> runs;  // per round
 };
 
-// One group of kPerGroup launches back to back on s: per-launch time.
+// One group of launches back to back on s: two untimed ones, so that the
+// timed ones start on a busy GPU as in a stream of work, then kPerGroup
+// timed; per-launch time.
 int time_group(DevState *st, const Candidate &c, bool inverse, const RaggedTiles &io,
                hipStream_t s, hipEvent_t e0, hipEvent_t e1, float *ms)
 {
-  constexpr int kPerGroup = 4;
+  constexpr int kLead = 2, kPerGroup = 6;
   const int form = inverse ? kFormCentred : kFormCanon;
   const LaunchShape ls{st->ncu, c.waves, c.oversub};
+  for (int i = 0; i < kLead; i++)
+    launch_mfma_ragged(c.shape, inverse, form, io, st->tables, ls, s);
   (void)hipEventRecord(e0, s);
   for (int i = 0; i < kPerGroup; i++)
     launch_mfma_ragged(c.shape, inverse, form, io, st->tables, ls, s);
@@ -121,9 +127,9 @@ int time_group(DevState *st, const Candidate &c, bool inverse, const RaggedTiles
 // The requests are rebased onto a scratch pool: with their own gaps when the
 // pool spans at most 1.25x their bytes (the kernels' cost does not depend on
 // the values), else back to back.  At most 2^20 polys of requests are timed
-// (a prefix of the table).  The candidates are timed in interleaved rounds,
-// each keeping its best group, so a clock ramp or a noisy neighbour affects
-// them alike.
+// (a prefix of the table).  The candidates are timed in interleaved rounds
+// and scored by their median round, so a clock ramp or a noisy neighbour
+// affects them alike and one lucky group does not decide.
 int autotune_requests(DevState *st, const tk_request *table, size_t nreq, int direction,
                       TuneVal *out, hipStream_t s)
 {
@@ -201,7 +207,7 @@ int autotune_requests(DevState *st, const tk_request *table, size_t nreq, int di
   for (int shape : {16, 32})
     for (int waves : {2, 4})
       for (int oversub : {4, 8, 12})
-        cand.push_back(Candidate{shape, waves, oversub, 1e30f});
+        cand.push_back(Candidate{shape, waves, oversub, 1e30f, {}});
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (rc == TK_OK)
     rc = record_hip(hipEventCreate(&e0));
@@ -209,15 +215,20 @@ int autotune_requests(DevState *st, const tk_request *table, size_t nreq, int di
     rc = record_hip(hipEventCreate(&e1));
   const bool inverse = direction != 0;
   // Warm-up (clocks, code objects), then interleaved rounds.
-  constexpr int kRounds = 4;
+  constexpr int kRounds = 5;
   for (int round = -1; round < kRounds && rc == TK_OK; round++)
     for (Candidate &c : cand) {
       float ms = 0;
       rc = time_group(st, c, inverse, c.shape == 32 ? io32 : io16, s, e0, e1, &ms);
       if (rc != TK_OK)
         break;
-      if (round >= 0 && ms < c.ms)
-        c.ms = ms;
+      if (round >= 0)
+        c.runs.push_back(ms);
+    }
+  for (Candidate &c : cand)
+    if (!c.runs.empty()) {
+      std::sort(c.runs.begin(), c.runs.end());
+      c.ms = c.runs[c.runs.size() / 2];
     }
   if (e0)
     (void)hipEventDestroy(e0);
@@ -252,8 +263,6 @@ int autotune_requests(DevState *st, const tk_request *table, size_t nreq, int di
 
 void plan_free(tk_ragged_plan *p)
 {
-  if (p->exec)
-    (void)hipGraphExecDestroy(p->exec);
   if (p->graph)
     (void)hipGraphDestroy(p->graph);
   if (p->dmem)
@@ -397,8 +406,6 @@ int tk_ragged_plan_create(tk_ragged_plan **plan, const tk_request *table, size_t
     const int erc = record_hip(hipStreamEndCapture(cap, &p->graph));
     rc = lrc ? lrc : erc;
   }
-  if (rc == TK_OK)
-    rc = record_hip(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
   (void)hipStreamDestroy(cap);
   if (rc) {
     plan_free(p);
@@ -408,18 +415,31 @@ int tk_ragged_plan_create(tk_ragged_plan **plan, const tk_request *table, size_t
   return TK_OK;
 }
 
+// One dispatch of the recorded persistent kernel.  A replay of the plan's
+// one-node graph would cost more: each graph launch adds 4-10 us of device
+// time on this runtime (an empty kernel: 2.4 us direct, 11.5 us as a graph;
+// a 190 us copy: +4 us; eight copies in one graph: +0 per copy), about 2.5%
+// of a config-4 pass.  The graph is for callers that compose plans into
+// their own graphs (tk_ragged_plan_graph), where that cost is paid once per
+// graph.
 int tk_ragged_plan_execute(tk_ragged_plan *plan, void *stream)
-{
-  if (!plan || !plan->exec)
-    return TK_EINVAL;
-  return record_hip(hipGraphLaunch(plan->exec, (hipStream_t)stream));
-}
-
-int tk_ragged_plan_launch(tk_ragged_plan *plan, void *stream)
 {
   if (!plan)
     return TK_EINVAL;
   return plan_launch(plan, (hipStream_t)stream);
+}
+
+int tk_ragged_plan_launch(tk_ragged_plan *plan, void *stream)
+{
+  return tk_ragged_plan_execute(plan, stream);
+}
+
+int tk_ragged_plan_graph(tk_ragged_plan *plan, void **graph)
+{
+  if (!plan || !graph || !plan->graph)
+    return TK_EINVAL;
+  *graph = plan->graph;
+  return TK_OK;
 }
 
 int tk_ragged_plan_info(const tk_ragged_plan *plan, int *method, int *polys_per_wave, int *grid,

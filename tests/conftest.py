@@ -1,3 +1,4 @@
+import ctypes
 import os
 import sys
 
@@ -69,3 +70,93 @@ def canon(a):
 def centred(a):
     c = np.mod(np.asarray(a, dtype=np.int32), 3329)
     return np.where(c > 1664, c - 3329, c).astype(np.int16)
+
+
+class Hip:
+    """The HIP runtime already loaded into this process (torch's), through
+    ctypes, so that pointers, streams and graphs are shared with libtkyber:
+    exact-size allocations and graph calls that torch does not expose."""
+
+    ALLOC = 2 << 20  # bytes per at_end allocation (a multiple of any page size)
+
+    def __init__(self):
+        import torch
+        torch.cuda.init()
+        path = None
+        with open("/proc/self/maps") as f:
+            for line in f:
+                if "libamdhip64.so" in line:
+                    path = line.split()[-1]
+                    break
+        assert path, "HIP runtime not loaded"
+        self.lib = ctypes.CDLL(path)
+        vp, sz, pvp = ctypes.c_void_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)
+        for name, args in (("hipMalloc", [pvp, sz]), ("hipFree", [vp]),
+                           ("hipHostMalloc", [pvp, sz, ctypes.c_uint]),
+                           ("hipHostFree", [vp]), ("hipMemcpy", [vp, vp, sz, ctypes.c_int]),
+                           ("hipDeviceSynchronize", []),
+                           ("hipGraphCreate", [pvp, ctypes.c_uint]),
+                           ("hipGraphDestroy", [vp]),
+                           ("hipGraphAddChildGraphNode", [pvp, vp, vp, sz, vp]),
+                           ("hipGraphInstantiate", [pvp, vp, vp, vp, sz]),
+                           ("hipGraphLaunch", [vp, vp]),
+                           ("hipGraphExecDestroy", [vp])):
+            getattr(self.lib, name).argtypes = args
+            getattr(self.lib, name).restype = ctypes.c_int
+        self.live = []
+
+    def alloc(self, nbytes, pinned=False):
+        p = ctypes.c_void_p()
+        rc = (self.lib.hipHostMalloc(ctypes.byref(p), nbytes, 0) if pinned
+              else self.lib.hipMalloc(ctypes.byref(p), nbytes))
+        assert rc == 0 and p.value, f"allocation of {nbytes} bytes failed: {rc}"
+        self.live.append((p.value, pinned))
+        return p.value
+
+    def at_end(self, nbytes, pinned=False, size=ALLOC):
+        """Address of nbytes that end at the last byte of a fresh allocation."""
+        assert nbytes <= size
+        return self.alloc(size, pinned) + size - nbytes
+
+    def put(self, dst, a):
+        a = np.ascontiguousarray(a)
+        assert self.lib.hipMemcpy(dst, a.ctypes.data, a.nbytes, 1) == 0
+
+    def get(self, src, shape, dtype=np.int16):
+        a = np.empty(shape, dtype)
+        assert self.lib.hipMemcpy(a.ctypes.data, src, a.nbytes, 2) == 0
+        return a
+
+    def sync(self):
+        assert self.lib.hipDeviceSynchronize() == 0
+
+    def instantiate(self, graph):
+        ex = ctypes.c_void_p()
+        assert self.lib.hipGraphInstantiate(ctypes.byref(ex), graph, None, None, 0) == 0
+        return ex.value
+
+    def compose(self, children):
+        """A new graph running the child graphs one after another."""
+        g = ctypes.c_void_p()
+        assert self.lib.hipGraphCreate(ctypes.byref(g), 0) == 0
+        prev = None
+        for child in children:
+            node = ctypes.c_void_p()
+            deps = (ctypes.c_void_p * 1)(prev) if prev else None
+            assert self.lib.hipGraphAddChildGraphNode(ctypes.byref(node), g, deps,
+                                                      1 if prev else 0, child) == 0
+            prev = node.value
+        return g.value
+
+    def free_all(self):
+        self.sync()
+        for p, pinned in self.live:
+            (self.lib.hipHostFree if pinned else self.lib.hipFree)(p)
+        self.live = []
+
+
+@pytest.fixture
+def hip(gpu):
+    h = Hip()
+    yield h
+    h.free_all()

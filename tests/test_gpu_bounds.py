@@ -21,70 +21,6 @@ pytestmark = pytest.mark.gpu
 
 N, Q = 256, 3329
 PB = N * 2  # bytes per poly
-ALLOC = 2 << 20  # bytes per device allocation (a multiple of any page size)
-
-
-class Hip:
-    """The HIP runtime already loaded into this process (torch's), so that
-    pointers are shared with libtkyber."""
-
-    def __init__(self):
-        import torch
-        torch.cuda.init()
-        path = None
-        with open("/proc/self/maps") as f:
-            for line in f:
-                if "libamdhip64.so" in line:
-                    path = line.split()[-1]
-                    break
-        assert path, "HIP runtime not loaded"
-        self.lib = ctypes.CDLL(path)
-        vp, sz = ctypes.c_void_p, ctypes.c_size_t
-        for name, args in (("hipMalloc", [ctypes.POINTER(vp), sz]), ("hipFree", [vp]),
-                           ("hipHostMalloc", [ctypes.POINTER(vp), sz, ctypes.c_uint]),
-                           ("hipHostFree", [vp]), ("hipMemcpy", [vp, vp, sz, ctypes.c_int]),
-                           ("hipDeviceSynchronize", [])):
-            getattr(self.lib, name).argtypes = args
-            getattr(self.lib, name).restype = ctypes.c_int
-        self.live = []
-
-    def alloc(self, nbytes, pinned=False):
-        p = ctypes.c_void_p()
-        rc = (self.lib.hipHostMalloc(ctypes.byref(p), nbytes, 0) if pinned
-              else self.lib.hipMalloc(ctypes.byref(p), nbytes))
-        assert rc == 0 and p.value, f"allocation of {nbytes} bytes failed: {rc}"
-        self.live.append((p.value, pinned))
-        return p.value
-
-    def at_end(self, nbytes, pinned=False, size=ALLOC):
-        """Address of nbytes that end at the last byte of a fresh allocation."""
-        assert nbytes <= size
-        return self.alloc(size, pinned) + size - nbytes
-
-    def put(self, dst, a):
-        a = np.ascontiguousarray(a)
-        assert self.lib.hipMemcpy(dst, a.ctypes.data, a.nbytes, 1) == 0
-
-    def get(self, src, shape, dtype=np.int16):
-        a = np.empty(shape, dtype)
-        assert self.lib.hipMemcpy(a.ctypes.data, src, a.nbytes, 2) == 0
-        return a
-
-    def sync(self):
-        assert self.lib.hipDeviceSynchronize() == 0
-
-    def free_all(self):
-        self.sync()
-        for p, pinned in self.live:
-            (self.lib.hipHostFree if pinned else self.lib.hipFree)(p)
-        self.live = []
-
-
-@pytest.fixture
-def hip(gpu):
-    h = Hip()
-    yield h
-    h.free_all()
 
 
 def _uniform(tk, npoly, seed):

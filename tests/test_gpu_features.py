@@ -68,26 +68,50 @@ def test_ragged_plan_small(tk, gpu, oracle, method):
     assert tk.lib().tk_ragged_plan_destroy(iplan) == 0
 
 
-def test_ragged_graph_replays(tk, gpu, oracle):
-    """Each replay of the captured graph matches the eager result; eager work
-    and synchronisation run between the replays."""
+def test_ragged_graph_replays(tk, gpu, oracle, hip):
+    """The plan's recorded graph, instantiated and replayed: each replay
+    matches the eager result, with eager work and synchronisation between the
+    replays; then the forward and inverse plans as child nodes of one graph
+    of the caller's."""
     import torch
     reqs, pool = tk.wl_ragged_layout(5000, 22)
     x = tk.wl_uniform(pool * N, 22).reshape(pool, N)
     d = _dev(x, gpu)
     plan = _make_plan(tk, d, reqs, 0, tk.METHOD_MFMA16)
+    iplan = _make_plan(tk, d, reqs, 1, tk.METHOD_MFMA32)
     want = x.copy()
     for off, n, _k, _kind in reqs:
         want[off:off + n] = canon(oracle.ntt(x[off:off + n]))
+    back = want.copy()
+    for off, n, _k, _kind in reqs:
+        back[off:off + n] = centred(oracle.invntt(want[off:off + n]))
+    g, gi = ctypes.c_void_p(), ctypes.c_void_p()
+    assert tk.lib().tk_ragged_plan_graph(plan, ctypes.byref(g)) == 0
+    assert tk.lib().tk_ragged_plan_graph(iplan, ctypes.byref(gi)) == 0
+    ex = hip.instantiate(g.value)
     src = _dev(x, gpu)
     for rep in range(4):
         d.copy_(src)
         tk.ntt(torch.empty((64, N), dtype=torch.int16, device=gpu).zero_())  # eager work
         torch.cuda.synchronize()
-        assert tk.lib().tk_ragged_plan_execute(plan, _stream()) == 0
+        assert hip.lib.hipGraphLaunch(ex, _stream()) == 0
         torch.cuda.synchronize()
         assert np.array_equal(d.cpu().numpy(), want), f"replay {rep}"
+    hip.lib.hipGraphExecDestroy(ex)
+    both = hip.compose([g.value, gi.value])
+    exb = hip.instantiate(both)
+    for rep in range(3):
+        d.copy_(src)
+        torch.cuda.synchronize()
+        assert hip.lib.hipGraphLaunch(exb, _stream()) == 0
+        torch.cuda.synchronize()
+        assert np.array_equal(d.cpu().numpy(), back), f"composed replay {rep}"
+        assert tk.lib().tk_ragged_plan_execute(plan, _stream()) == 0  # eager, in between
+        torch.cuda.synchronize()
+    hip.lib.hipGraphExecDestroy(exb)
+    hip.lib.hipGraphDestroy(both)
     tk.lib().tk_ragged_plan_destroy(plan)
+    tk.lib().tk_ragged_plan_destroy(iplan)
 
 
 @pytest.mark.parametrize("method", ["auto", "mfma16", "mfma32"])
