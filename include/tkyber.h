@@ -140,6 +140,31 @@ int tk_ragged_plan_info(const tk_ragged_plan *plan, int *method, int *polys_per_
                         int *oversub, size_t *npoly_total);
 int tk_ragged_plan_destroy(tk_ragged_plan *plan);
 
+/* ---- ragged transform without allocation ------------------------------- */
+
+/* No counterpart in the reference.  For callers that cannot allocate (code
+ * under hipGraph capture, tight serving loops, harnesses that forbid device
+ * allocation): the transform of tk_ragged_plan_create + tk_ragged_plan_execute
+ * in one call, with no plan.  The request table (host memory, as for a
+ * plan) is copied on stream into a fixed device buffer of this library,
+ * expanded there into the table of tiles of 16 polys, and the persistent
+ * kernel runs over it (MFMA16 tiles, the linear batches' launch shape);
+ * forward output canonical, inverse centred, in place.
+ *   - Allocates nothing.  tk_init() must have run on the current device
+ *     (it sets the buffer up); TK_EINVAL otherwise.
+ *   - At most TK_NOALLOC_MAX_REQUESTS requests and TK_NOALLOC_MAX_TILES
+ *     tiles (a request takes ceil(npoly / 16)); larger tables give TK_EINVAL.
+ *   - The buffer (128 KiB + 2 MiB) is one per device and holds the last
+ *     call's contents until the next: calls that overlap on one device
+ *     (other streams, other threads) race.
+ *   - The table is the source of an asynchronous copy on stream: it must stay
+ *     unchanged until the stream has passed this call.
+ *   - Requests must not overlap one another; this call does not check it. */
+#define TK_NOALLOC_MAX_REQUESTS 8192
+#define TK_NOALLOC_MAX_TILES 131072
+int tk_ragged_transform_noalloc(const tk_request *table, size_t nreq, int direction,
+                                void *stream);
+
 /* ---- fused module matrix-vector product -------------------------------- */
 
 /* For each of ninst instances: t_i = invntt( sum_j A_hat[i][j] o ntt(s_j) ),

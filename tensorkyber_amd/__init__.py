@@ -123,6 +123,7 @@ def _declare(lib):
                                  ctypes.POINTER(sz)], i),
         "tk_ragged_plan_destroy": ([vp], i),
         "tk_ragged_plan_graph": ([vp, ctypes.POINTER(vp)], i),
+        "tk_ragged_transform_noalloc": ([vp, sz, i, vp], i),
         "matvec_ntt_fused": ([vp, vp, vp, i, sz, vp], i),
         "tk_autotune": ([sz, i, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(i), vp], i),
     }

@@ -6,7 +6,6 @@
 #include <string.h>
 
 #include <algorithm>
-#include <map>
 #include <mutex>
 #include <utility>
 
@@ -85,9 +84,26 @@ int init_device(DevState **out)
     }
     st.ncu = prop.multiProcessorCount;
     st.device = dev;
+    int rc = noalloc_init(&st);
+    if (rc) {
+      (void)hipFree(d);
+      return rc;
+    }
     st.tables = d;
   }
   *out = &st;
+  return TK_OK;
+}
+
+int device_ready(DevState **out)
+{
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices)
+    return TK_ENODEV;
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!g_dev[dev].tables)
+    return TK_EINVAL;
+  *out = &g_dev[dev];
   return TK_OK;
 }
 
@@ -143,20 +159,28 @@ int device_form(int form)
   }
 }
 
+// A fixed table rather than a map: launches after the first of each kernel
+// allocate nothing (the no-allocation entry point relies on it).
 int resident_per_cu(const void *kernel, int block_threads)
 {
+  struct Entry {
+    const void *kernel;
+    int block, n;
+  };
+  constexpr int kEntries = 1024;
   static std::mutex mu;
-  static std::map<std::pair<const void *, int>, int> cache;
+  static Entry cache[kEntries];
+  static int used = 0;
   std::lock_guard<std::mutex> lk(mu);
-  const auto key = std::make_pair(kernel, block_threads);
-  auto it = cache.find(key);
-  if (it != cache.end())
-    return it->second;
+  for (int i = 0; i < used; i++)
+    if (cache[i].kernel == kernel && cache[i].block == block_threads)
+      return cache[i].n;
   int n = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, block_threads, 0) != hipSuccess ||
       n < 1)
     n = 1;
-  cache[key] = n;
+  if (used < kEntries)
+    cache[used++] = Entry{kernel, block_threads, n};
   return n;
 }
 

@@ -73,6 +73,10 @@ int launch_butterfly(bool inverse, int form, int16_t *out, const int16_t *in, si
 int launch_mfma(int shape, bool inverse, int form, int16_t *out, const int16_t *in, size_t npoly,
                 const DeviceTables *tab, const LaunchShape &ls, hipStream_t stream);
 
+// Caches the occupancy of the ragged kernels of one shape and workgroup
+// size, so that their later launches allocate nothing.
+void prepare_ragged_launch(int shape, int waves);
+
 // io.tiles must have been cut for shape's tile height (16 or 32 polys).
 int launch_mfma_ragged(int shape, bool inverse, int form, const RaggedTiles &io,
                        const DeviceTables *tab, const LaunchShape &ls, hipStream_t stream);

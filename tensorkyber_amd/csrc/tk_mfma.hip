@@ -361,6 +361,23 @@ int launch_mfma(int shape, bool inverse, int form, int16_t *out, const int16_t *
   return launch_any(shape, inverse, form, io, tab, ls, stream);
 }
 
+void prepare_ragged_launch(int shape, int waves)
+{
+  // The forms ragged plans use: canonical forward, centred inverse.
+  auto one = [&](const void *k) { (void)resident_per_cu(k, 64 * waves); };
+  if (shape == 32) {
+    one(waves == 2 ? (const void *)k_mfma32<kFormCanon, false, 2, RaggedTiles>
+                   : (const void *)k_mfma32<kFormCanon, false, 4, RaggedTiles>);
+    one(waves == 2 ? (const void *)k_mfma32<kFormCentred, true, 2, RaggedTiles>
+                   : (const void *)k_mfma32<kFormCentred, true, 4, RaggedTiles>);
+  } else {
+    one(waves == 2 ? (const void *)k_mfma16<kFormCanon, false, 2, RaggedTiles>
+                   : (const void *)k_mfma16<kFormCanon, false, 4, RaggedTiles>);
+    one(waves == 2 ? (const void *)k_mfma16<kFormCentred, true, 2, RaggedTiles>
+                   : (const void *)k_mfma16<kFormCentred, true, 4, RaggedTiles>);
+  }
+}
+
 int launch_mfma_ragged(int shape, bool inverse, int form, const RaggedTiles &io,
                        const DeviceTables *tab, const LaunchShape &ls, hipStream_t stream)
 {

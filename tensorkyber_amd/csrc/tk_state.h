@@ -20,10 +20,20 @@ struct DevState {
   int device = -1;
   int ncu = 0;
   DeviceTables *tables = nullptr;
+  // Device address of this device's copy of the no-allocation entry
+  // point's static scratch array (tk_noalloc.hip).
+  void *noalloc_scratch = nullptr;
 };
 
 // Uploads the tables to the current device on first use.  Thread-safe.
 int init_device(DevState **out);
+// The current device's state if init_device has run on it (TK_EINVAL if
+// not), without initialising anything: no allocation.
+int device_ready(DevState **out);
+// Per-device set-up of the no-allocation entry point, from init_device with
+// the device current: the scratch array's device address and the launch
+// shapes it will use.
+int noalloc_init(DevState *st);
 int record_hip(hipError_t e);
 int after_launch();
 bool aligned16(const void *p);

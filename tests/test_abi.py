@@ -263,3 +263,31 @@ def test_fused_plan_rejects_cross_request_overlap(tk):
     assert create() == tk.EINVAL
     reqs[1].out0 = reqs[1].x                               # its own input, exactly: allowed
     _created_or_nodev(tk, L, create(), plan, L.tk_fused_plan_destroy)
+
+
+def test_noalloc_validation(tk):
+    """The no-allocation ragged transform rejects tables beyond its fixed
+    buffer, and bad requests, before touching the device."""
+    L = tk.lib()
+    buf = (ctypes.c_int16 * (N * 64 + 16))()
+    al = (ctypes.addressof(buf) + 15) & ~15
+    one = (tk.TkRequest * 1)()
+    one[0].base, one[0].npoly, one[0].k, one[0].kind = al, 64, 3, 0
+    f = L.tk_ragged_transform_noalloc
+    assert f(None, 1, 0, None) == tk.EINVAL
+    assert f(ctypes.addressof(one), 0, 0, None) == tk.EINVAL
+    assert f(ctypes.addressof(one), 1, 2, None) == tk.EINVAL             # direction
+    one[0].k = 5
+    assert f(ctypes.addressof(one), 1, 0, None) == tk.EINVAL             # rank
+    one[0].k, one[0].base = 3, al + 2
+    assert f(ctypes.addressof(one), 1, 0, None) == tk.EINVAL             # misaligned
+    one[0].base, one[0].npoly = al, 16 * 131072 + 1
+    assert f(ctypes.addressof(one), 1, 0, None) == tk.EINVAL             # 2^17 + 1 tiles
+    one[0].npoly = 0
+    assert f(ctypes.addressof(one), 1, 0, None) == tk.EINVAL             # nothing to do
+    many = (tk.TkRequest * 8193)()
+    for r in range(8193):
+        many[r].base, many[r].npoly, many[r].k, many[r].kind = al, 1, 2, 0
+    assert f(ctypes.addressof(many), 8193, 0, None) == tk.EINVAL         # 8193 requests
+    one[0].npoly = 64
+    assert f(ctypes.addressof(one), 1, 0, None) in (tk.ENODEV, tk.EINVAL, tk.OK)

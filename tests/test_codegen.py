@@ -81,10 +81,18 @@ _WIDE_STORE = re.compile(r"^(buffer|global|flat)_store_dwordx(3|4)\b.*?v\[(\d+):
 _VALU_DST = re.compile(r"^v_\S+\s+v\[?(\d+)(?::(\d+))?")
 
 
+def _functions(asm):
+    """(name, body) of every function in the assembly (data symbols, such as
+    device arrays, have labels too but no .Lfunc_end)."""
+    funcs = set(re.findall(r"^\s*\.type\s+(\w+),\s*@function", asm, re.M))
+    for m in re.finditer(r"^(_Z\w+):", asm, re.M):
+        if m.group(1) in funcs:
+            yield m.group(1), asm[m.end():asm.index(".Lfunc_end", m.end())]
+
+
 def _wide_store_hazards(asm):
     found, stores = [], 0
-    for m in re.finditer(r"^(_Z\w+):", asm, re.M):
-        body = asm[m.end():asm.index(".Lfunc_end", m.end())]
+    for name, body in _functions(asm):
         ins = [l.strip() for l in body.split("\n")
                if l.startswith("\t") and not l.strip().startswith((".", ";"))]
         for j, line in enumerate(ins[:-1]):
@@ -97,7 +105,7 @@ def _wide_store_hazards(asm):
             if d:
                 a, b = int(d.group(1)), int(d.group(2) or d.group(1))
                 if a <= hi and b >= lo:
-                    found.append(f"{m.group(1)}: {line} || {ins[j + 1]}")
+                    found.append(f"{name}: {line} || {ins[j + 1]}")
     return found, stores
 
 
@@ -112,7 +120,7 @@ def test_wide_store_data_not_overwritten_next(compiled):
 
 def test_scan_detects_the_pattern():
     """The scan itself flags a store followed at once by a write of its data."""
-    asm = ("_Zfake:\n\tbuffer_store_dwordx4 v[8:11], v43, s[20:23], s65 offen nt\n"
+    asm = ("\t.type\t_Zfake,@function\n_Zfake:\n\tbuffer_store_dwordx4 v[8:11], v43, s[20:23], s65 offen nt\n"
            "\tv_bfe_i32 v9, v4, 0, 16\n\ts_endpgm\n.Lfunc_end0:\n")
     found, stores = _wide_store_hazards(asm)
     assert stores == 1 and len(found) == 1
