@@ -30,7 +30,7 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_POLY_PASS = 1024  # 512 B read + 512 B written per poly per transform
 
 
-EXTRAS = ("basemul", "ragged", "matvec", "packed", "kpke", "sched", "host")
+EXTRAS = ("c1", "c2", "basemul", "ragged", "matvec", "packed", "kpke", "sched", "host")
 
 
 def parse():
@@ -49,6 +49,12 @@ def parse():
                     help="comma list of extra sections to run: " + ", ".join(EXTRAS))
     ap.add_argument("--extras-only", action="store_true",
                     help="profiling run: only the selected extra sections, printed as JSON")
+    ap.add_argument("--profile-launches", action="store_true",
+                    help="profiling run: --steps launches of each transform kernel and of a "
+                         "copy of the batch, nothing timed")
+    ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="write profiles/traffic.json from two rocprofv3 --pmc runs of "
+                         "--profile-launches (FETCH_SIZE, WRITE_SIZE); no GPU needed")
     ap.add_argument("--kernels-only", action="store_true",
                     help="profiling run: launch each kernel `steps` times plus a same-size "
                          "device copy, print per-kernel times, no CPU leg or parity check")
@@ -197,6 +203,91 @@ def hip_graph_runtime():
                        ("hipGraphLaunch", [vp, vp]), ("hipGraphExecDestroy", [vp])):
         getattr(lib, name).argtypes, getattr(lib, name).restype = args, ctypes.c_int
     return lib
+
+
+def run_stats(ms, npoly):
+    """The reference's statistics of timed runs (ntt_bench.cu:428-488): mean,
+    median and sample standard deviation of the per-run throughput, and the
+    median latency."""
+    ms = np.asarray(ms, dtype=np.float64)
+    tp = npoly / (ms / 1e3)
+    return {"mean_poly_s": float(tp.mean()), "median_poly_s": float(np.median(tp)),
+            "sd_poly_s": float(tp.std(ddof=1)) if tp.size > 1 else 0.0,
+            "median_latency_ms": float(np.median(ms))}
+
+
+def bench_config_rows(tk, torch, config, runs, methods=("mfma16", "mfma32", "butterfly")):
+    """Configurations 1 and 2 as the reference runs them (ntt_bench.cu:632-663,
+    548-587): per run, the batch is re-staged from a pristine device copy, then
+    the forward NTT (in place) and the inverse NTT (in place) are each timed
+    alone between two events with a synchronisation after each; `runs` timed
+    runs after 2 warm-ups.  Per direction: mean / median / sd poly/s and median
+    latency; round trip npoly / (t_fwd + t_inv) per run.  Bit-exact: the last
+    run's forward and inverse outputs against the oracle on every poly.  The
+    reference C code (ntt() then invntt() per poly) is timed on the host on all
+    threads and on one."""
+    from oracle.oracle import Oracle, Reference, reference_available
+    x = tk.workload(config)
+    npoly = x.shape[0]
+    src = torch.from_numpy(x).to("cuda:0")
+    buf = torch.empty_like(src)
+    stream = torch.cuda.current_stream()
+    o = Oracle()
+    want_f = np.mod(o.ntt(x).astype(np.int32), 3329).astype(np.int16)
+    inv = np.mod(o.invntt(want_f).astype(np.int32), 3329)
+    want_i = np.where(inv > 1664, inv - 3329, inv).astype(np.int16)
+    out = {"workload": f"configuration {config}: {npoly} polys uniform [0,q), seed {config}",
+           "npoly": npoly, "runs": runs}
+    for m in methods:
+        def one():
+            buf.copy_(src)
+            tk.ntt(buf, method=m)
+            tk.invntt(buf, method=m)
+        warm(torch, one, 2)
+        ms_f, ms_i = [], []
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+        for _ in range(runs):
+            buf.copy_(src)
+            e[0].record(stream)
+            tk.ntt(buf, method=m)
+            e[1].record(stream)
+            torch.cuda.synchronize()
+            ms_f.append(e[0].elapsed_time(e[1]))
+            fwd = buf.cpu().numpy() if _ == runs - 1 else None
+            e[2].record(stream)
+            tk.invntt(buf, method=m)
+            e[3].record(stream)
+            torch.cuda.synchronize()
+            ms_i.append(e[2].elapsed_time(e[3]))
+        rt = npoly / ((np.asarray(ms_f) + np.asarray(ms_i)) / 1e3)
+        out[m] = {"ntt": run_stats(ms_f, npoly), "invntt": run_stats(ms_i, npoly),
+                  "roundtrip": {"mean_poly_s": float(rt.mean()),
+                                "median_poly_s": float(np.median(rt))},
+                  "roofline_frac_rt_median": float(np.median(rt)) * 2 * BYTES_PER_POLY_PASS /
+                  1e9 / HBM_PEAK_GBS,
+                  "bit_exact": bool(np.array_equal(fwd, want_f) and
+                                    np.array_equal(buf.cpu().numpy(), want_i))}
+    impl = Reference() if reference_available() else Oracle()
+    cpu = {"kind": "reference" if reference_available() else "port"}
+    for label, nthreads in (("all_threads", impl.max_threads()), ("one_thread", 1)):
+        a = x.copy()
+        reps = 3 if nthreads > 1 else 1
+        ms_f, ms_i = [], []
+        for _ in range(reps):
+            a[:] = x
+            t0 = time.perf_counter()
+            impl.ntt_inplace(a, nthreads)
+            t1 = time.perf_counter()
+            impl.invntt_inplace(a, nthreads)
+            t2 = time.perf_counter()
+            ms_f.append((t1 - t0) * 1e3)
+            ms_i.append((t2 - t1) * 1e3)
+        rt = npoly / ((np.asarray(ms_f) + np.asarray(ms_i)) / 1e3)
+        cpu[label] = {"threads": nthreads, "runs": reps, "ntt": run_stats(ms_f, npoly),
+                      "invntt": run_stats(ms_i, npoly),
+                      "roundtrip_median_poly_s": float(np.median(rt))}
+    out["cpu_reference"] = cpu
+    return out
 
 
 def bench_ragged(tk, torch, steps):
@@ -588,22 +679,126 @@ def check_parity(tk, torch, method, x, buf_fwd, buf_rt):
     return bool(ok_f and np.array_equal(inv, want))
 
 
+TRAFFIC_JSON = os.path.join(ROOT, "profiles", "traffic.json")
+CSRC = os.path.join(ROOT, "tensorkyber_amd", "csrc")
+# A plain device copy of the batch (torch's copy_ of 512 MiB) calibrates the
+# counters: its bytes read and written are known exactly.
+COPY_TAG = "copy"
+
+
+def csrc_digest():
+    """SHA-256 (first 16 hex digits) of the kernel sources, file name and
+    contents in name order: ties a traffic measurement to the code measured."""
+    import hashlib
+    h = hashlib.sha256()
+    for name in sorted(os.listdir(CSRC)):
+        path = os.path.join(CSRC, name)
+        if os.path.isfile(path):
+            h.update(name.encode() + b"\0")
+            with open(path, "rb") as f:
+                h.update(f.read())
+    return h.hexdigest()[:16]
+
+
 def load_traffic(kernel, npoly):
-    """HBM bytes per launch of `kernel`, measured with rocprofv3 FETCH_SIZE and
-    WRITE_SIZE at 2^20 polys (profiles/traffic.json); None at other sizes."""
+    """HBM bytes per launch of `kernel` at 2^20 polys from profiles/traffic.json
+    (two rocprofv3 --pmc runs, FETCH_SIZE and WRITE_SIZE, calibrated by the
+    copy), and a note; None when the file was measured on other sources."""
     if npoly != 1 << 20:
-        return None
-    path = os.path.join(ROOT, "profiles", "traffic.json")
+        return None, "traffic is measured at 2^20 polys only"
     try:
-        with open(path) as f:
-            return json.load(f).get(kernel)
+        with open(TRAFFIC_JSON) as f:
+            t = json.load(f)
     except (OSError, ValueError):
-        return None
+        return None, "profiles/traffic.json missing"
+    if t.get("csrc_digest") != csrc_digest():
+        return None, (f"profiles/traffic.json was measured on kernel sources "
+                      f"{t.get('csrc_digest')}, these are {csrc_digest()}")
+    v = t.get("kernels", {}).get(kernel)
+    return (v["bytes_per_launch"] if v else None), f"rocprofv3 FETCH_SIZE/WRITE_SIZE, sources " \
+                                                    f"{t['csrc_digest']}"
+
+
+# Kernel name -> bench kernel label (forward canonical, inverse centred, as
+# ntt_batch / invntt_batch launch them).
+KERNEL_LABELS = (("k_mfma16<1, false, 4, tk::LinearTiles>", "mfma16_fwd"),
+                 ("k_mfma16<2, true, 4, tk::LinearTiles>", "mfma16_inv"),
+                 ("k_mfma32<1, false, 4, tk::LinearTiles>", "mfma32_fwd"),
+                 ("k_mfma32<2, true, 4, tk::LinearTiles>", "mfma32_inv"),
+                 ("k_ntt_butterfly", "butterfly_fwd"), ("k_invntt_butterfly", "butterfly_inv"))
+
+
+def kernel_label(name):
+    for tag, label in KERNEL_LABELS:
+        if tag in name:
+            return label
+    return COPY_TAG if COPY_TAG in name.lower() else None
+
+
+def profile_launches(tk, torch, npoly, reps):
+    """A short program for rocprofv3: `reps` launches of each transform kernel
+    and of a plain copy of the batch, nothing else timed or checked."""
+    x = tk.workload(3, npoly)
+    src = torch.from_numpy(x).to("cuda:0")
+    buf = torch.empty_like(src)
+    for m in ("mfma16", "mfma32", "butterfly"):
+        for _ in range(reps):
+            tk.ntt(src, out=buf, method=m)
+        for _ in range(reps):
+            tk.invntt(buf, method=m)
+        torch.cuda.synchronize()
+        print(f"[bench] profiled {m} x {reps}", file=sys.stderr, flush=True)
+    for _ in range(reps):
+        buf.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"profiled": npoly, "reps": reps, "csrc_digest": csrc_digest()}))
+
+
+def traffic_from(fetch_dir, write_dir, npoly):
+    """profiles/traffic.json from the counter_collection CSVs of two
+    rocprofv3 --pmc runs of profile_launches (FETCH_SIZE, then WRITE_SIZE, in
+    KiB per dispatch).  Each counter is scaled by what it reads for the plain
+    copy of the batch, whose bytes are known (npoly * 512 read and written)."""
+    import csv
+    import glob
+
+    def per_kernel(d, counter):
+        vals = {}
+        for path in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            with open(path) as f:
+                for row in csv.DictReader(f):
+                    if row.get("Counter_Name") != counter:
+                        continue
+                    label = kernel_label(row.get("Kernel_Name", ""))
+                    if label:
+                        vals.setdefault(label, []).append(float(row["Counter_Value"]))
+        return {k: float(np.median(v)) for k, v in vals.items()}
+
+    fetch, write = per_kernel(fetch_dir, "FETCH_SIZE"), per_kernel(write_dir, "WRITE_SIZE")
+    kib = npoly * 512 / 1024
+    f_scale, w_scale = kib / fetch[COPY_TAG], kib / write[COPY_TAG]
+    kernels = {}
+    for label in sorted(set(fetch) & set(write) - {COPY_TAG}):
+        b = (fetch[label] * f_scale + write[label] * w_scale) * 1024
+        kernels[label] = {"fetch_kib": fetch[label], "write_kib": write[label],
+                          "bytes_per_launch": int(round(b)),
+                          "vs_algorithmic": b / (npoly * BYTES_PER_POLY_PASS)}
+    out = {"csrc_digest": csrc_digest(), "npoly": npoly,
+           "copy_calibration": {"fetch_kib": fetch[COPY_TAG], "write_kib": write[COPY_TAG],
+                                "fetch_scale": f_scale, "write_scale": w_scale},
+           "kernels": kernels}
+    with open(TRAFFIC_JSON, "w") as f:
+        json.dump(out, f, indent=1)
+        f.write("\n")
+    print(json.dumps(out))
 
 
 def run_extras(tk, torch, args):
     """The sections beyond the headline, as selected by --extras."""
     sections = {
+        "c1": ("config1", "configuration 1 rows", lambda: bench_config_rows(
+            tk, torch, 1, 30, ("butterfly", "mfma16"))),
+        "c2": ("config2", "configuration 2 rows", lambda: bench_config_rows(tk, torch, 2, 30)),
         "basemul": ("basemul", "basemul_batch", lambda: bench_basemul(tk, torch, args.steps)),
         "ragged": ("config4_ragged", "config 4 ragged plan", lambda: bench_ragged(tk, torch, args.steps)),
         "matvec": ("config5_matvec", "config 5 fused matvec", lambda: bench_matvec(tk, torch, args.steps)),
@@ -631,6 +826,9 @@ def main():
               "exchange, so an N-GPU figure is N single-GPU runs; run with --gpus 1",
               file=sys.stderr)
         sys.exit(2)
+    if args.traffic_from:
+        traffic_from(*args.traffic_from, args.npoly)
+        return
     import torch
     import tensorkyber_amd as tk
     tk.load_library()
@@ -639,6 +837,10 @@ def main():
     rc = tk.lib().tk_init()
     if rc != 0:
         raise SystemExit(f"tk_init failed: {rc}")
+
+    if args.profile_launches:
+        profile_launches(tk, torch, args.npoly, args.steps)
+        return
 
     if args.extras_only:
         print(json.dumps(run_extras(tk, torch, args)))
@@ -690,6 +892,7 @@ def main():
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         cpu = cpu_baseline(x)
 
+    traffic, traffic_note = load_traffic(kname, npoly)
     line = {
         "metric": "NTT+iNTT poly/s at 2^20 polys (MFMA vs butterfly, bit-exact); % HBM roofline",
         "value": value,
@@ -717,7 +920,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": load_traffic(kname, npoly),
+            "traffic": traffic,
+            "traffic_note": traffic_note,
         },
         "rt_fraction_of_roofline": value * 2 * BYTES_PER_POLY_PASS / 1e9 / HBM_PEAK_GBS,
         "bit_exact": bit_exact,

@@ -121,6 +121,12 @@ class _Batch:
     def roundtrip(self, polys, nthreads: int = 0) -> np.ndarray:
         return self._call("roundtrip_batch", polys, nthreads)
 
+    def ntt_inplace(self, a: np.ndarray, nthreads: int = 0) -> None:
+        getattr(self.lib, self._prefix + "ntt_batch")(_ptr(a), a.size // N, nthreads)
+
+    def invntt_inplace(self, a: np.ndarray, nthreads: int = 0) -> None:
+        getattr(self.lib, self._prefix + "invntt_batch")(_ptr(a), a.size // N, nthreads)
+
     def roundtrip_inplace(self, a: np.ndarray, nthreads: int = 0) -> None:
         getattr(self.lib, self._prefix + "roundtrip_batch")(_ptr(a), a.size // N, nthreads)
 
