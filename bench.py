@@ -754,24 +754,39 @@ def profile_launches(tk, torch, npoly, reps):
     print(json.dumps({"profiled": npoly, "reps": reps, "csrc_digest": csrc_digest()}))
 
 
-def traffic_from(fetch_dir, write_dir, npoly):
-    """profiles/traffic.json from the counter_collection CSVs of two
-    rocprofv3 --pmc runs of profile_launches (FETCH_SIZE, then WRITE_SIZE, in
-    KiB per dispatch).  Each counter is scaled by what it reads for the plain
-    copy of the batch, whose bytes are known (npoly * 512 read and written)."""
+def counter_rows(d, counter):
+    """(kernel name, value) of every dispatch's `counter` in a rocprofv3
+    output directory: its SQLite databases (counters_collection view) or,
+    with --output-format csv, its counter_collection CSVs."""
     import csv
     import glob
+    import sqlite3
+    rows = []
+    for path in glob.glob(os.path.join(d, "**", "*.db"), recursive=True):
+        con = sqlite3.connect(path)
+        try:
+            rows += con.execute("select kernel_name, value from counters_collection "
+                                "where counter_name = ?", (counter,)).fetchall()
+        finally:
+            con.close()
+    for path in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+        with open(path) as f:
+            rows += [(r["Kernel_Name"], r["Counter_Value"]) for r in csv.DictReader(f)
+                     if r.get("Counter_Name") == counter]
+    return [(name, float(v)) for name, v in rows]
 
+
+def traffic_from(fetch_dir, write_dir, npoly):
+    """profiles/traffic.json from two rocprofv3 --pmc runs of profile_launches
+    (FETCH_SIZE, then WRITE_SIZE, in KiB per dispatch).  Each counter is
+    scaled by what it reads for the plain copy of the batch, whose bytes are
+    known (npoly * 512 read and written)."""
     def per_kernel(d, counter):
         vals = {}
-        for path in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
-            with open(path) as f:
-                for row in csv.DictReader(f):
-                    if row.get("Counter_Name") != counter:
-                        continue
-                    label = kernel_label(row.get("Kernel_Name", ""))
-                    if label:
-                        vals.setdefault(label, []).append(float(row["Counter_Value"]))
+        for name, v in counter_rows(d, counter):
+            label = kernel_label(name)
+            if label:
+                vals.setdefault(label, []).append(v)
         return {k: float(np.median(v)) for k, v in vals.items()}
 
     fetch, write = per_kernel(fetch_dir, "FETCH_SIZE"), per_kernel(write_dir, "WRITE_SIZE")
