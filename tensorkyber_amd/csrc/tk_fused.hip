@@ -41,10 +41,11 @@
 // the inverse matrix without the Montgomery factor (m16_inv1p).
 //
 // Each sum is two v_dot2_i32_i16 per pair and source j: A_hat's packed pair
-// (a0, a1) against (b0, gamma*b1 mod 2q) and against (b1, b0).  A_hat may hold
-// any int16 and b, gamma*b1 lie in [0, 2q), so |sum| <= k * 32768 * (q-1 +
-// 2q-1) < 1.31e9 < 2^31 for k <= 4; it is reduced once per output
-// coefficient.
+// (a0, a1) against (b0, gamma*b1 mod 2q) and against (b1, b0).  The forward
+// transforms leave b in [0, 2q) (no final fold: gamma * b1 < 2q * q is then
+// reduced with the 32-bit Barrett step), A_hat may hold any int16 and
+// gamma*b1 lies in [0, 2q), so |sum| <= k * 32768 * (2q-1 + 2q-1) < 1.75e9 <
+// 2^31 for k <= 4; it is reduced once per output coefficient.
 #include <type_traits>
 
 #include "tk_launch.h"
@@ -54,8 +55,10 @@ namespace tk {
 namespace {
 
 constexpr int kTileInst = 16;  // instances per tile: the MFMA16 poly width
-// Multiple of q above every |sum|: 4 * 32768 * (3328 + 6657) < kSumOffset.
-constexpr uint32_t kSumOffset = (4u * 32768u * (3328u + 6657u) / 3329u + 1u) * 3329u;
+// Multiple of q above every |sum|: 4 * 32768 * (6657 + 6657) < kSumOffset,
+// and kSumOffset + |sum| < 2^32 (red32's domain).
+constexpr uint32_t kSumOffset = (4u * 32768u * (6657u + 6657u) / 3329u + 1u) * 3329u;
+static_assert((uint64_t)kSumOffset + 4ull * 32768ull * 13314ull < (1ull << 32), "red32 domain");
 
 // [0, 2q) representative of x < 2^24.
 __device__ __forceinline__ uint32_t red24(uint32_t x)
@@ -349,7 +352,7 @@ __global__ __launch_bounds__((64 * fused_waves<MODE, K>())) void k_fused(
     wave_sync();
     pass_m16<kFormInter>(pslab, g, f1lo, f1hi, fb1);
     wave_sync();
-    pass_c16<kFormCanon>(pslab, g, f2lo, f2hi, s_bf2);
+    pass_c16<kFormInter>(pslab, g, f2lo, f2hi, s_bf2);
     wave_sync();
   };
 
@@ -395,7 +398,7 @@ __global__ __launch_bounds__((64 * fused_waves<MODE, K>())) void k_fused(
         }
         pass_m16_packed<SBITS>(pslab, g, pv, f1pklo, f1pkhi, fb1pk);
         wave_sync();
-        pass_c16<kFormCanon>(pslab, g, f2lo, f2hi, s_bf2);
+        pass_c16<kFormInter>(pslab, g, f2lo, f2hi, s_bf2);
         wave_sync();
       }
       if constexpr (MODE == kFusedKeygen) {
@@ -435,7 +438,7 @@ __global__ __launch_bounds__((64 * fused_waves<MODE, K>())) void k_fused(
 #pragma unroll
           for (int u = 0; u < 4; u++) {
             const uint32_t bu = (uint32_t)b[u];
-            const uint32_t x = __builtin_amdgcn_perm(red24((bu >> 16) * gin[u]), bu, 0x05040100u);
+            const uint32_t x = __builtin_amdgcn_perm(red32((bu >> 16) * gin[u]), bu, 0x05040100u);
             const uint32_t y = __builtin_amdgcn_alignbit(bu, bu, 16);
             r0[u] = dot2((uint32_t)a_cur[j][u], x, r0[u]);
             r1[u] = dot2((uint32_t)a_cur[j][u], y, r1[u]);
@@ -521,33 +524,50 @@ __global__ __launch_bounds__((64 * fused_waves<MODE, K>())) void k_fused(
 }
 
 constexpr int kDecryptWaves = 4;
+constexpr int kDecryptWideWaves = 16;
 
 // Decrypt, one wave per tile: w = reduce(v - invntt(sum_j s_hat_j o
 // ntt(u_j))).  Decrypt has one output row, so a wave forms it alone: for
 // each j it stages u_j of its 16 instances into its slab, forward-transforms
 // it and adds s_hat_j o u_hat_j (s_hat_j streamed from HBM) into the row it
 // holds in registers, packed; then it inverse-transforms the row in the slab
-// and writes w.  The waves of a workgroup share only the bias tables, so the
-// tile loop has no workgroup barrier.  gamma * b1 is formed where it is
-// used, as in the fused kernel's rows.
+// and writes w.  The waves of a workgroup share only the bias tables (and,
+// wide, the forward matrices), so the tile loop has no workgroup barrier.
+// gamma * b1 is formed where it is used, as in the fused kernel's rows.
 //
 // The row's partial sums are reduced per j to [0, 2q) and added as packed
 // int16 pairs (below 8q for K <= 4: no carry between the halves); the
 // inverse's first pass takes inputs below 8q, as the fused kernel's rows.
 //
-// Four waves per SIMD (at most 128 VGPRs; the LDS of four workgroups holds
-// the same 16 waves per CU).  The table-driven variant keeps a 64-bit
-// pointer per lane and access and runs at three rather than spill.
-template <int K, bool TABLE>
-__global__ __launch_bounds__(64 * kDecryptWaves)
+// Two shapes, both four waves per SIMD (at most 128 VGPRs):
+//   - WIDE (contiguous buffers): one workgroup of 16 waves per CU, holding
+//     the forward transform's per-block matrices in LDS (16 KB, shared by
+//     the 16 waves; 153,600 B of LDS in all).  The forward transform then
+//     issues no vector memory load, so s_hat_j's loads go out before it and
+//     are in flight for its whole length: vector memory loads complete in
+//     order, and a matrix fetched from memory inside the transform would wait
+//     for them.  Waves stride over the tiles (the grid is one workgroup per
+//     CU).
+//   - four-wave workgroups fetching the matrices from the (cache-resident)
+//     table, s_hat_j loaded after the forward transform: the table-driven
+//     variant, whose 64-bit pointer rows take the LDS; it runs at three
+//     waves per SIMD rather than spill.
+template <int K, bool TABLE, bool WIDE>
+__global__ __launch_bounds__(64 * (WIDE ? kDecryptWideWaves : kDecryptWaves))
 __attribute__((amdgpu_waves_per_eu(TABLE ? 3 : 4))) void k_decrypt(
     const FusedIo io, const DeviceTables *__restrict__ tab)
 {
-  constexpr int P = kTileInst, W = kDecryptWaves;
+  static_assert(!(TABLE && WIDE), "the table-driven variant is four-wave");
+  constexpr int P = kTileInst, W = WIDE ? kDecryptWideWaves : kDecryptWaves;
   __shared__ const int16_t *s_ip[TABLE ? W : 1][TABLE ? P : 1][8];
   __shared__ __align__(16) int16_t s_slab[W][P * kStride];
   __shared__ __align__(16) int32_t s_bf2[16][16];
   __shared__ __align__(16) int32_t s_bi2[16][16];
+  // Wide: the forward per-block matrices and both first-pass matrices,
+  // [block][limb][lane] (fwd2), [limb][lane] (fwd1, inv1p).
+  __shared__ __align__(16) uint64_t s_f2[WIDE ? 16 : 1][2][64];
+  __shared__ __align__(16) uint64_t s_p1[WIDE ? 2 : 1][2][64];
+  __shared__ __align__(16) int32_t s_b1[WIDE ? 2 : 1][16];  // bias_fwd1, bias_inv1p
   const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int lane = threadIdx.x & 63, g = lane >> 4, tt = lane & 15;
 
@@ -555,10 +575,39 @@ __attribute__((amdgpu_waves_per_eu(TABLE ? 3 : 4))) void k_decrypt(
     s_bf2[i >> 4][i & 15] = tab->bias_fwd2[i >> 4][i & 15];
     s_bi2[i >> 4][i & 15] = tab->bias_inv2[i >> 4][i & 15];
   }
+  if constexpr (WIDE) {
+    for (int i = threadIdx.x; i < 16 * 2 * 64; i += 64 * W)
+      s_f2[i >> 7][(i >> 6) & 1][i & 63] =
+          *reinterpret_cast<const uint64_t *>(tab->m16_fwd2[i >> 7][(i >> 6) & 1][i & 63]);
+    for (int i = threadIdx.x; i < 2 * 64; i += 64 * W) {
+      s_p1[0][i >> 6][i & 63] = *reinterpret_cast<const uint64_t *>(tab->m16_fwd1[0][i >> 6][i & 63]);
+      s_p1[1][i >> 6][i & 63] = *reinterpret_cast<const uint64_t *>(tab->m16_inv1p[0][i >> 6][i & 63]);
+    }
+    for (int i = threadIdx.x; i < 16; i += 64 * W) {
+      s_b1[0][i] = tab->bias_fwd1[0][i];
+      s_b1[1][i] = tab->bias_inv1p[0][i];
+    }
+  }
   __syncthreads();
 
-  const long f1lo = ld_a(tab->m16_fwd1[0], 0, lane), f1hi = ld_a(tab->m16_fwd1[0], 1, lane);
-  const long i1lo = ld_a(tab->m16_inv1p[0], 0, lane), i1hi = ld_a(tab->m16_inv1p[0], 1, lane);
+  // First-pass matrices: held in registers, or (wide) read from LDS at each
+  // pass, which keeps their 8 VGPRs free for s_hat_j in flight.
+  const long f1lo_r = WIDE ? 0 : ld_a(tab->m16_fwd1[0], 0, lane);
+  const long f1hi_r = WIDE ? 0 : ld_a(tab->m16_fwd1[0], 1, lane);
+  const long i1lo_r = WIDE ? 0 : ld_a(tab->m16_inv1p[0], 0, lane);
+  const long i1hi_r = WIDE ? 0 : ld_a(tab->m16_inv1p[0], 1, lane);
+  auto p1 = [&](int which, int limb, long reg) {
+    if constexpr (WIDE)
+      return (long)s_p1[which][limb][lane];
+    else
+      return reg;
+  };
+  auto b1 = [&](int which, const v4i &reg) {
+    if constexpr (WIDE)
+      return *reinterpret_cast<const v4i *>(&s_b1[which][4 * g]);
+    else
+      return reg;
+  };
   const v4i fb1 = *reinterpret_cast<const v4i *>(&tab->bias_fwd1[0][4 * g]);
   const v4i ib1 = *reinterpret_cast<const v4i *>(&tab->bias_inv1p[0][4 * g]);
   const __amdgpu_buffer_rsrc_t trs = __builtin_amdgcn_make_buffer_rsrc(
@@ -568,8 +617,18 @@ __attribute__((amdgpu_waves_per_eu(TABLE ? 3 : 4))) void k_decrypt(
     return __builtin_bit_cast(long, __builtin_amdgcn_raw_buffer_load_b64(trs, loff, off, 0));
   };
   constexpr int kF2 = (int)offsetof(DeviceTables, m16_fwd2), kI2 = (int)offsetof(DeviceTables, m16_inv2);
-  auto f2lo = [&](int m) { return mat(kF2 + m * 1024); };
-  auto f2hi = [&](int m) { return mat(kF2 + m * 1024 + 512); };
+  auto f2lo = [&](int m) {
+    if constexpr (WIDE)
+      return (long)s_f2[m][0][lane];
+    else
+      return mat(kF2 + m * 1024);
+  };
+  auto f2hi = [&](int m) {
+    if constexpr (WIDE)
+      return (long)s_f2[m][1][lane];
+    else
+      return mat(kF2 + m * 1024 + 512);
+  };
   auto i2lo = [&](int m) { return mat(kI2 + m * 1024); };
   auto i2hi = [&](int m) { return mat(kI2 + m * 1024 + 512); };
   // Lane (., ch) holds coefficients 8ch .. 8ch+7 of a poly: pairs u = 0..3,
@@ -595,8 +654,11 @@ __attribute__((amdgpu_waves_per_eu(TABLE ? 3 : 4))) void k_decrypt(
                              (int)role_stride<kFusedDecrypt, K>(rl), lane);
   };
 
+  // Tile t goes to workgroup t mod grid, wave (t / grid) mod W: a batch of
+  // fewer tiles than wave slots spreads over every CU.
   const size_t ntiles = (io.ninst + P - 1) / P, nwaves = (size_t)gridDim.x * W;
-  for (size_t tile = (size_t)blockIdx.x * W + w; tile < ntiles; tile += nwaves) {
+  for (size_t tile = WIDE ? (size_t)w * gridDim.x + blockIdx.x : (size_t)blockIdx.x * W + w;
+       tile < ntiles; tile += nwaves) {
     i0 = tile * P;
     nvalid = (int)(io.ninst - i0 < (size_t)P ? io.ninst - i0 : P);
     if constexpr (TABLE) {
@@ -636,15 +698,23 @@ __attribute__((amdgpu_waves_per_eu(TABLE ? 3 : 4))) void k_decrypt(
         const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
         *reinterpret_cast<v4i *>(slab + pp * kStride + ch * 8) = un[it];
       }
-      wave_sync();
-      pass_m16<kFormInter>(pslab, g, f1lo, f1hi, fb1);
-      wave_sync();
-      pass_c16<kFormCanon>(pslab, g, f2lo, f2hi, s_bf2);
-      wave_sync();
       v4i sn[P / 2];
+      if constexpr (WIDE) {
+        // In flight during the whole forward transform, which loads nothing.
 #pragma unroll
-      for (int it = 0; it < P / 2; it++)
-        sn[it] = sm.load(it, j);
+        for (int it = 0; it < P / 2; it++)
+          sn[it] = sm.load(it, j);
+      }
+      wave_sync();
+      pass_m16<kFormInter>(pslab, g, p1(0, 0, f1lo_r), p1(0, 1, f1hi_r), b1(0, fb1));
+      wave_sync();
+      pass_c16<kFormInter>(pslab, g, f2lo, f2hi, s_bf2);
+      wave_sync();
+      if constexpr (!WIDE) {
+#pragma unroll
+        for (int it = 0; it < P / 2; it++)
+          sn[it] = sm.load(it, j);
+      }
 #pragma unroll
       for (int it = 0; it < P / 2; it++) {
         const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
@@ -652,7 +722,7 @@ __attribute__((amdgpu_waves_per_eu(TABLE ? 3 : 4))) void k_decrypt(
 #pragma unroll
         for (int u = 0; u < 4; u++) {
           const uint32_t bu = (uint32_t)b[u];
-          const uint32_t gb1 = red24((bu >> 16) * gam[u]);
+          const uint32_t gb1 = red32((bu >> 16) * gam[u]);
           const uint32_t x = __builtin_amdgcn_perm(gb1, bu, 0x05040100u);
           const uint32_t y = __builtin_amdgcn_alignbit(bu, bu, 16);
           const uint32_t r = pack2(red32((uint32_t)dot2((uint32_t)sn[it][u], x, 0) + kSumOffset),
@@ -680,17 +750,24 @@ __attribute__((amdgpu_waves_per_eu(TABLE ? 3 : 4))) void k_decrypt(
     wave_sync();
     pass_c16<kFormInter>(pslab, g, i2lo, i2hi, s_bi2);
     wave_sync();
-    pass_m16<kFormCentred>(pslab, g, i1lo, i1hi, ib1);
-    wave_sync();
-
     // w = reduce(v - row); every load of v goes out before the first store
-    // (w may alias v).
+    // (w may alias v).  Wide, they go out before the inverse's last pass,
+    // whose matrices are in registers.
     const auto va = role(rAdd0);
     const auto oa = role(rOut0);
-    const v4i zero = {0, 0, 0, 0};
+    if constexpr (WIDE) {
 #pragma unroll
-    for (int it = 0; it < P / 2; it++)
-      un[it] = va.load(it, 0);
+      for (int it = 0; it < P / 2; it++)
+        un[it] = va.load(it, 0);
+    }
+    pass_m16<kFormCentred>(pslab, g, p1(1, 0, i1lo_r), p1(1, 1, i1hi_r), b1(1, ib1));
+    wave_sync();
+    const v4i zero = {0, 0, 0, 0};
+    if constexpr (!WIDE) {
+#pragma unroll
+      for (int it = 0; it < P / 2; it++)
+        un[it] = va.load(it, 0);
+    }
 #pragma unroll
     for (int it = 0; it < P / 2; it++) {
       const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
@@ -757,22 +834,25 @@ int launch_fused(int k, int mode, int s_bits, const FusedIo &io, const DeviceTab
     case kFusedDecrypt: {
       if (s_bits)
         return -1;
-      auto dec = [&](auto kernel) {
-        const size_t cap = (size_t)ncu *
-                           resident_per_cu((const void *)kernel, 64 * kDecryptWaves) *
-                           kFusedOversub;
-        const size_t wgs = (ntiles + kDecryptWaves - 1) / kDecryptWaves;
-        hipLaunchKernelGGL(kernel, dim3((unsigned)(wgs < cap ? wgs : cap)),
-                           dim3(64 * kDecryptWaves), 0, stream, io, tab);
+      // Wide: one workgroup of 16 waves per CU, the waves striding over the
+      // tiles.  Table-driven: four-wave workgroups, up to kFusedOversub x the
+      // resident ones.
+      auto dec = [&](auto kernel, int waves, int oversub) {
+        const size_t cap =
+            (size_t)ncu * resident_per_cu((const void *)kernel, 64 * waves) * oversub;
+        const size_t wgs = waves == kDecryptWideWaves ? ntiles : (ntiles + waves - 1) / waves;
+        hipLaunchKernelGGL(kernel, dim3((unsigned)(wgs < cap ? wgs : cap)), dim3(64 * waves), 0,
+                           stream, io, tab);
         return 0;
       };
+      constexpr int kW = kDecryptWideWaves, kT = kDecryptWaves;
       switch (k * 2 + (io.inst ? 1 : 0)) {
-        case 4: return dec(k_decrypt<2, false>);
-        case 5: return dec(k_decrypt<2, true>);
-        case 6: return dec(k_decrypt<3, false>);
-        case 7: return dec(k_decrypt<3, true>);
-        case 8: return dec(k_decrypt<4, false>);
-        case 9: return dec(k_decrypt<4, true>);
+        case 4: return dec(k_decrypt<2, false, true>, kW, 1);
+        case 5: return dec(k_decrypt<2, true, false>, kT, kFusedOversub);
+        case 6: return dec(k_decrypt<3, false, true>, kW, 1);
+        case 7: return dec(k_decrypt<3, true, false>, kT, kFusedOversub);
+        case 8: return dec(k_decrypt<4, false, true>, kW, 1);
+        case 9: return dec(k_decrypt<4, true, false>, kT, kFusedOversub);
         default: return -1;
       }
     }
