@@ -731,6 +731,8 @@ __attribute__((amdgpu_waves_per_eu(TABLE ? 3 : 4))) void k_decrypt(
         }
         if constexpr (!kLast)
           un[it] = um.load(it, j + 1);
+        else if constexpr (WIDE)
+          un[it] = role(rAdd0).load(it, 0);  // v, for the epilogue
         __builtin_amdgcn_sched_barrier(0);
       }
     };
@@ -751,15 +753,10 @@ __attribute__((amdgpu_waves_per_eu(TABLE ? 3 : 4))) void k_decrypt(
     pass_c16<kFormInter>(pslab, g, i2lo, i2hi, s_bi2);
     wave_sync();
     // w = reduce(v - row); every load of v goes out before the first store
-    // (w may alias v).  Wide, they go out before the inverse's last pass,
-    // whose matrices are in registers.
+    // (w may alias v).  Wide, they went out during the last product, into
+    // the registers s_hat's chunks freed.
     const auto va = role(rAdd0);
     const auto oa = role(rOut0);
-    if constexpr (WIDE) {
-#pragma unroll
-      for (int it = 0; it < P / 2; it++)
-        un[it] = va.load(it, 0);
-    }
     pass_m16<kFormCentred>(pslab, g, p1(1, 0, i1lo_r), p1(1, 1, i1hi_r), b1(1, ib1));
     wave_sync();
     const v4i zero = {0, 0, 0, 0};
