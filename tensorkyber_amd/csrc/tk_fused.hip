@@ -523,6 +523,188 @@ __global__ __launch_bounds__((64 * fused_waves<MODE, K>())) void k_fused(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Wide matvec: the fused product (contiguous buffers, int16 s) with one
+// workgroup of several K-wave tiles per CU, so that the transform matrices
+// of both directions fit once in LDS for all of them.  The transforms then
+// issue no vector memory load, and since vector memory loads complete in
+// order, loads issued before a transform stay in flight through it instead
+// of being waited for at its first matrix fetch:
+//   - the first kWidePre chunks of each wave's matrix row go out before its
+//     forward transform (phase 1) and are ready when phase 2 starts; phase 2
+//     keeps kWidePre chunks in flight;
+//   - the next tile's s_w goes out before the inverse transform (phase 3),
+//     into the registers the row results free.
+// The tiles of a workgroup go through the phases together (workgroup
+// barriers); tile t of iteration it belongs to sub-tile (t / grid) mod S of
+// workgroup t mod grid, so a batch of fewer tiles than CUs still spreads.
+// ---------------------------------------------------------------------------
+// biases of both per-block passes, fwd2 + inv2 matrices, fwd1 + inv1p
+// matrices and their biases
+constexpr int kWideLdsTables = 2 * 1024 + 2 * 16384 + 2 * 1024 + 2 * 64;
+template <int K>
+constexpr int wide_tiles()
+{
+  constexpr int by_lds = (163840 - kWideLdsTables) / (K * kTileInst * kStride * 2);
+  return by_lds < 16 / K ? by_lds : 16 / K;
+}
+constexpr int kWidePre = 3;  // matrix-row chunks in flight
+
+template <int K>
+__global__ __launch_bounds__(64 * K * wide_tiles<K>()) __attribute__((amdgpu_waves_per_eu(4)))
+void k_matvec_wide(const FusedIo io, const DeviceTables *__restrict__ tab)
+{
+  constexpr int P = kTileInst, S = wide_tiles<K>(), W = K * S;
+  __shared__ __align__(16) int16_t s_slab[S][K][P * kStride];
+  __shared__ __align__(16) int32_t s_bf2[16][16];
+  __shared__ __align__(16) int32_t s_bi2[16][16];
+  __shared__ __align__(16) uint64_t s_m2[2][16][2][64];  // [fwd2, inv2][block][limb][lane]
+  __shared__ __align__(16) uint64_t s_m1[2][2][64];      // [fwd1, inv1p][limb][lane]
+  __shared__ __align__(16) int32_t s_b1[2][16];          // bias_fwd1, bias_inv1p
+  const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int st = w / K, wi = w % K;  // sub-tile, wave within it
+  const int lane = threadIdx.x & 63, g = lane >> 4, tt = lane & 15;
+
+  for (int i = threadIdx.x; i < 256; i += 64 * W) {
+    s_bf2[i >> 4][i & 15] = tab->bias_fwd2[i >> 4][i & 15];
+    s_bi2[i >> 4][i & 15] = tab->bias_inv2[i >> 4][i & 15];
+  }
+  for (int i = threadIdx.x; i < 16 * 2 * 64; i += 64 * W) {
+    s_m2[0][i >> 7][(i >> 6) & 1][i & 63] =
+        *reinterpret_cast<const uint64_t *>(tab->m16_fwd2[i >> 7][(i >> 6) & 1][i & 63]);
+    s_m2[1][i >> 7][(i >> 6) & 1][i & 63] =
+        *reinterpret_cast<const uint64_t *>(tab->m16_inv2[i >> 7][(i >> 6) & 1][i & 63]);
+  }
+  for (int i = threadIdx.x; i < 2 * 64; i += 64 * W) {
+    s_m1[0][i >> 6][i & 63] = *reinterpret_cast<const uint64_t *>(tab->m16_fwd1[0][i >> 6][i & 63]);
+    s_m1[1][i >> 6][i & 63] = *reinterpret_cast<const uint64_t *>(tab->m16_inv1p[0][i >> 6][i & 63]);
+  }
+  for (int i = threadIdx.x; i < 16; i += 64 * W) {
+    s_b1[0][i] = tab->bias_fwd1[0][i];
+    s_b1[1][i] = tab->bias_inv1p[0][i];
+  }
+  __syncthreads();
+
+  // Every matrix and bias is read from LDS at each pass, keeping registers
+  // for the loads in flight.
+  auto m1 = [&](int which, int limb) { return (long)s_m1[which][limb][lane]; };
+  auto b1 = [&](int which) { return *reinterpret_cast<const v4i *>(&s_b1[which][4 * g]); };
+  auto f2lo = [&](int m) { return (long)s_m2[0][m][0][lane]; };
+  auto f2hi = [&](int m) { return (long)s_m2[0][m][1][lane]; };
+  auto i2lo = [&](int m) { return (long)s_m2[1][m][0][lane]; };
+  auto i2hi = [&](int m) { return (long)s_m2[1][m][1][lane]; };
+  uint32_t gin[4];
+  {
+    const uint32_t ga = tab->gamma[2 * (lane & 31)], gb = tab->gamma[2 * (lane & 31) + 1];
+    gin[0] = ga;
+    gin[1] = (uint32_t)kQ - ga;
+    gin[2] = gb;
+    gin[3] = (uint32_t)kQ - gb;
+  }
+
+  int16_t *const slab = s_slab[st][wi];
+  int16_t *const pslab = slab + tt * kStride;
+  const size_t ntiles = (io.ninst + P - 1) / P, G = gridDim.x, per_iter = (size_t)S * G;
+  auto tile_of = [&](size_t base) { return base + (size_t)st * G + blockIdx.x; };
+  auto nvalid_of = [&](size_t tile) {
+    const size_t i0 = tile * P;
+    return tile < ntiles ? (int)(io.ninst - i0 < (size_t)P ? io.ninst - i0 : P) : 0;
+  };
+  auto rt = [&](int role, size_t tile, int nvalid) {
+    return RoleTile<false>(role_base(io, role), tile * P, nvalid,
+                           (int)role_stride<kFusedMatvec, K>(role), lane);
+  };
+  // s_w of this sub-tile's first tile; later ones are prefetched in phase 3.
+  v4i sx[P / 2];
+  {
+    const size_t t0 = tile_of(0);
+    const auto xr = rt(rX, t0, nvalid_of(t0));
+#pragma unroll
+    for (int it = 0; it < P / 2; it++)
+      sx[it] = xr.load(it, wi);
+  }
+
+  for (size_t base = 0; base < ntiles; base += per_iter) {
+    const size_t tile = tile_of(base);
+    const int nvalid = nvalid_of(tile);  // 0: an idle sub-tile (it still meets the barriers)
+    // 1. s_w -> slab, the row's first chunks in flight, forward transform.
+    const auto mr = rt(rMat, tile, nvalid);
+    v4i a[kWidePre][K];
+    wave_sync();
+#pragma unroll
+    for (int it = 0; it < P / 2; it++) {
+      const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
+      *reinterpret_cast<v4i *>(slab + pp * kStride + ch * 8) = sx[it];
+    }
+#pragma unroll
+    for (int c = 0; c < kWidePre; c++)
+#pragma unroll
+      for (int j = 0; j < K; j++)
+        a[c][j] = mr.load(c, wi * K + j);
+    if (nvalid) {
+      wave_sync();
+      pass_m16<kFormInter>(pslab, g, m1(0, 0), m1(0, 1), b1(0));
+      wave_sync();
+      pass_c16<kFormInter>(pslab, g, f2lo, f2hi, s_bf2);
+    }
+    __syncthreads();
+
+    // 2. Row wi: kWidePre chunks of the matrix row in flight.
+    uint32_t res[P / 2][4];
+#pragma unroll
+    for (int it = 0; it < P / 2 && nvalid; it++) {
+      const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
+      v4i *cur = a[it % kWidePre];
+      int r0[4] = {0, 0, 0, 0}, r1[4] = {0, 0, 0, 0};
+#pragma unroll
+      for (int j = 0; j < K; j++) {
+        const v4i b = *reinterpret_cast<const v4i *>(s_slab[st][j] + pp * kStride + ch * 8);
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const uint32_t bu = (uint32_t)b[u];
+          const uint32_t x = __builtin_amdgcn_perm(red32((bu >> 16) * gin[u]), bu, 0x05040100u);
+          const uint32_t y = __builtin_amdgcn_alignbit(bu, bu, 16);
+          r0[u] = dot2((uint32_t)cur[j][u], x, r0[u]);
+          r1[u] = dot2((uint32_t)cur[j][u], y, r1[u]);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        res[it][u] =
+            pack2(red32((uint32_t)r0[u] + kSumOffset), red32((uint32_t)r1[u] + kSumOffset));
+      if (it + kWidePre < P / 2)
+#pragma unroll
+        for (int j = 0; j < K; j++)
+          cur[j] = mr.load(it + kWidePre, wi * K + j);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    __syncthreads();  // every wave is done reading the slabs
+
+    // 3. The row into the slab, the next tile's s_w in flight, inverse, t.
+#pragma unroll
+    for (int it = 0; it < P / 2; it++) {
+      const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
+      *reinterpret_cast<v4i *>(slab + pp * kStride + ch * 8) =
+          v4i{(int)res[it][0], (int)res[it][1], (int)res[it][2], (int)res[it][3]};
+    }
+    {
+      const size_t nt = tile + per_iter;
+      const auto xr = rt(rX, nt, nvalid_of(nt));
+#pragma unroll
+      for (int it = 0; it < P / 2; it++)
+        sx[it] = xr.load(it, wi);
+    }
+    if (nvalid) {
+      wave_sync();
+      pass_c16<kFormInter>(pslab, g, i2lo, i2hi, s_bi2);
+      wave_sync();
+      pass_m16<kFormCentred>(pslab, g, m1(1, 0), m1(1, 1), b1(1));
+      wave_sync();
+      role_out<P>(slab, rt(rOut0, tile, nvalid), wi, lane);
+    }
+  }
+}
+
 constexpr int kDecryptWaves = 4;
 constexpr int kDecryptWideWaves = 16;
 
@@ -813,9 +995,21 @@ int launch_fused(int k, int mode, int s_bits, const FusedIo &io, const DeviceTab
   };
   if (io.inst && s_bits)
     return -1;
+  // Wide matvec: one workgroup per CU (its LDS), persistent over the tiles.
+  auto wide = [&](auto kernel, int waves) {
+    const size_t grid = ntiles < (size_t)ncu ? ntiles : (size_t)ncu;
+    hipLaunchKernelGGL(kernel, dim3((unsigned)grid), dim3(64 * waves), 0, stream, io, tab);
+    return 0;
+  };
   switch (mode) {
     case kFusedMatvec: {
       const integral_constant<int, kFusedMatvec> mc{};
+      if (s_bits == 0 && !io.inst)
+        switch (k) {
+          case 2: return wide(k_matvec_wide<2>, 2 * wide_tiles<2>());
+          case 3: return wide(k_matvec_wide<3>, 3 * wide_tiles<3>());
+          default: break;  // rank 4: three tiles per CU, fewer waves than k_fused keeps
+        }
       switch (s_bits) {
         case 0: return by_table(mc);
         case 2: return with_k(mc, integral_constant<int, 2>{}, Lin{});
