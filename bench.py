@@ -686,17 +686,22 @@ CSRC = os.path.join(ROOT, "tensorkyber_amd", "csrc")
 COPY_TAG = "copy"
 
 
+# The sources the traffic-measured kernels (the linear MFMA16 / MFMA32 and
+# butterfly transforms, their launch shapes and tables) are built from.
+TRAFFIC_SOURCES = ("tk_api.hip", "tk_butterfly.hip", "tk_device.h", "tk_internal.h",
+                   "tk_launch.h", "tk_mfma.hip", "tk_mfma_core.h", "tk_tables.cpp")
+
+
 def csrc_digest():
-    """SHA-256 (first 16 hex digits) of the kernel sources, file name and
-    contents in name order: ties a traffic measurement to the code measured."""
+    """SHA-256 (first 16 hex digits) of the traffic-measured kernels' sources,
+    file name and contents in name order: ties a traffic measurement to the
+    code measured."""
     import hashlib
     h = hashlib.sha256()
-    for name in sorted(os.listdir(CSRC)):
-        path = os.path.join(CSRC, name)
-        if os.path.isfile(path):
-            h.update(name.encode() + b"\0")
-            with open(path, "rb") as f:
-                h.update(f.read())
+    for name in TRAFFIC_SOURCES:
+        h.update(name.encode() + b"\0")
+        with open(os.path.join(CSRC, name), "rb") as f:
+            h.update(f.read())
     return h.hexdigest()[:16]
 
 

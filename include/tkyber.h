@@ -329,7 +329,7 @@ int matvec_ntt_fused_packed(int16_t *t, const int16_t *A_hat, const uint8_t *s_p
  * ragged-plan kernel on scratch data of npoly polys (at most 2^20) laid out
  * as one request, on the current device, in interleaved rounds, and returns
  * the fastest -- unless it beats the linear batches' shape (MFMA16, 4 waves,
- * 8x) by less than 2%, which is then kept: method (TK_METHOD_MFMA16 or
+ * 8x) by less than 5%, which is then kept: method (TK_METHOD_MFMA16 or
  * TK_METHOD_MFMA32), waves per workgroup and the oversubscription factor.
  * Plans with method TK_METHOD_AUTO run the same search on their own
  * requests.  Results are cached per device, direction, size octave and
