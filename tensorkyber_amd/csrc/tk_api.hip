@@ -555,7 +555,7 @@ int ntt_batch_packed(int16_t *out, const uint8_t *packed, size_t npoly, int bits
   if ((rc = init_device(&st)))
     return rc;
   if (launch_mfma_packed(m == TK_METHOD_MFMA32 ? 32 : 16, bits, device_form(f), out, packed, npoly,
-                         st->tables, LaunchShape{st->ncu, 4, kPackedOversub}, (hipStream_t)stream))
+                         st->tables, LaunchShape{st->ncu, packed_waves(bits), kPackedOversub}, (hipStream_t)stream))
     return TK_ENOTSUP;
   return after_launch();
 }

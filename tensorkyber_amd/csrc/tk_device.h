@@ -120,6 +120,19 @@ __device__ __forceinline__ v16i mfma32(v4i a, v4i b, v16i c)
   return __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, c, 0, 0, 0);
 }
 
+// Wait until every vector memory operation the wave has issued is done
+// (s_waitcnt vmcnt(0), expcnt and lgkmcnt left at their maximum).
+//
+// Kernels call it once, after the loads that precede their tile loop.  The
+// compiler places one wait per use of a loaded register, valid on every path
+// into it: at the loop header that covers both the entry edge, where a
+// prologue load may be the last operation issued, and the back edge, where
+// the previous tile's stores follow the loads.  Vector memory operations
+// complete in issue order, so a wait sized for the entry edge (vmcnt(0))
+// also waits for those stores on every iteration.  Waiting here, once, keeps
+// the loop's waits sized for the back edge.
+__device__ __forceinline__ void vm_drain() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 // All lanes of the wave have finished their LDS writes before any lane reads
 // what another lane wrote.  One wave owns each LDS region, so no workgroup
 // barrier is needed.

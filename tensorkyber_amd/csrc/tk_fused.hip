@@ -623,6 +623,7 @@ void k_matvec_wide(const FusedIo io, const DeviceTables *__restrict__ tab)
     for (int it = 0; it < P / 2; it++)
       sx[it] = xr.load(it, wi);
   }
+  vm_drain();  // the loop's waits for s_w then count the previous tile's stores
 
   for (size_t base = 0; base < ntiles; base += per_iter) {
     const size_t tile = tile_of(base);
@@ -699,9 +700,12 @@ void k_matvec_wide(const FusedIo io, const DeviceTables *__restrict__ tab)
       pass_c16<kFormInter>(pslab, g, i2lo, i2hi, s_bi2);
       wave_sync();
       pass_m16<kFormCentred>(pslab, g, m1(1, 0), m1(1, 1), b1(1));
-      wave_sync();
-      role_out<P>(slab, rt(rOut0, tile, nvalid), wi, lane);
     }
+    // Also for an idle sub-tile, whose descriptor has size 0 and drops the
+    // stores: every path into the next iteration then has the same
+    // operations in flight, and the wait for s_w stays behind the stores.
+    wave_sync();
+    role_out<P>(slab, rt(rOut0, tile, nvalid), wi, lane);
   }
 }
 

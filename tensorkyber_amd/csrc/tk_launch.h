@@ -49,15 +49,18 @@ struct PackedTiles {
 // calculator).  Each wave strides over tiles; a factor above 1 lets the
 // dispatcher hand late workgroups to whichever CU frees up first, which
 // balances the load.  Linear batches use 8 for 16-poly tiles (2^20 polys:
-// 182-183 us per pass, against 190 at 4 and 192 at 2) and 12 for 32-poly
-// tiles (180-183 / 184-188 us forward / inverse, against 188-190 / 190-192
-// at 8; the inverse slows again from 16).  Packed batches use 2: their waves load few bytes per tile, and
-// on MI355X 2^20 packed polys take 137-140 us (2 bits) and 142-145 us
-// (3 bits) at 1 or 2, against 146 and 153 us at 8 and 168 and 164 us at 32.
-// Ragged plans let the autotuner choose, starting from the linear shape.
+// 181-183 us per pass, against 187-188 at 4 and 189-191 at 16) and 12 for
+// 32-poly tiles (180-183 / 184-188 us forward / inverse, against 188-190 /
+// 190-192 at 8; the inverse slows again from 16).  Packed batches load few
+// bytes per tile and use 1, with workgroups of 2 waves for 2-bit input and
+// 4 for 3 and 4 bits: 2^20 polys take 126-127 us at 2 bits (128-129 with 4
+// waves, 129-130 at 2x) and 135-136 us at 3 bits (137-138 with 2 waves);
+// 4x took 141-143 us at 2 bits.  Ragged plans let the autotuner choose,
+// starting from the linear shape.
 constexpr int kLinearOversub = 8;
 constexpr int kLinearOversub32 = 12;
-constexpr int kPackedOversub = 2;
+constexpr int kPackedOversub = 1;
+constexpr int packed_waves(int bits) { return bits == 2 ? 2 : 4; }
 struct LaunchShape {
   int ncu;
   int waves;

@@ -163,7 +163,9 @@ struct PackedBits<PackedTiles<BITS>> : std::integral_constant<int, BITS> {
 
 // At least three waves per SIMD (at most 168 VGPRs): without the bound the
 // scheduler trades a wave of occupancy for longer-lived staging registers.
-// The int16 and 2-bit packed kernels fit 128 VGPRs and run four.
+// The int16 kernels take 152 VGPRs (three waves: the tile swap holds the next
+// tile's loads and half the stores' data at once), the packed ones 128
+// (four).
 template <int FORM, bool INV, int WAVES, class Tiles>
 __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(3))) void k_mfma16(Tiles io,
                                                         const DeviceTables *__restrict__ tab)
@@ -199,26 +201,44 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(3)))
   int16_t *Pp = L + t * kStride;
   // Grid-stride over tiles.  Latency is hidden by the other resident waves
   // (three or four per SIMD) and by launching several times more workgroups
-  // than are resident, which also balances load across CUs.  The passes are
-  // hidden: the kernel runs at the speed of its staging alone.  A register
-  // prefetch of the next tile (a wave of occupancy) measured no faster.
-  // The next tile's description (for ragged tiles a scalar load of the
-  // table) is issued at the top of each iteration, a tile ahead of its use.
+  // than are resident, which also balances load across CUs.  The next
+  // tile's description (for ragged tiles a scalar load of the table) is
+  // issued at the top of each iteration, a tile ahead of its use.
+  //
+  // Vector memory operations complete in issue order, so no wait for a load
+  // may fall shortly after the previous tile's stores: it would wait for
+  // them as well (on MI355X a write is acknowledged microseconds after its
+  // store issues).  The next tile's loads therefore go out before the
+  // current tile's stores: int16 tiles swap (tile_swap_rsrc16: loads,
+  // stores, then the waits for the loads), packed tiles load into the
+  // staging registers as soon as the first pass has consumed them.  Both
+  // measured faster than loading after the stores: 180-182 against 183 us
+  // per 2^20-poly int16 pass, 126-130 against 138-140 us for the 2-bit
+  // packed forward.  vm_drain keeps the loop's waits from covering the
+  // prologue's loads.
   const size_t ntiles = tile_count<P>(io);
   const size_t stride = (size_t)gridDim.x * WAVES;
   size_t tile = (size_t)blockIdx.x * WAVES + w;
   TileIo cur = tile_io<P>(io, tile < ntiles ? tile : 0);
   StageRegs<P, Tiles> sr;
   stage_load<P>(io, tile, sr, lane);
+  if constexpr (PB == 0)
+    if (tile < ntiles)
+      stage_in<P>(io, cur, L, sr, lane);
+  vm_drain();
   for (; tile < ntiles; tile += stride) {
-    const TileIo nxt = tile_io<P>(io, tile + stride < ntiles ? tile + stride : tile);
-    stage_in<P>(io, cur, L, sr, lane);
+    const bool more = tile + stride < ntiles;
+    const TileIo nxt = tile_io<P>(io, more ? tile + stride : tile);
+    if constexpr (PB != 0)
+      stage_in<P>(io, cur, L, sr, lane);
     wave_sync();
     if (!INV) {
-      if constexpr (PB != 0)
+      if constexpr (PB != 0) {
         pass_m16_packed<PB>(Pp, g, sr.v, a1lo, a1hi, bias1);
-      else
+        stage_load<P>(io, tile + stride, sr, lane);
+      } else {
         pass_m16<kFormInter>(Pp, g, a1lo, a1hi, bias1);
+      }
       wave_sync();
       pass_c16<FORM>(Pp, g, [&](int m) { return a2lo[m]; }, [&](int m) { return a2hi[m]; },
                      s_bias);
@@ -229,8 +249,11 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(3)))
       pass_m16<FORM>(Pp, g, a1lo, a1hi, bias1);
     }
     wave_sync();
-    stage_load<P>(io, tile + stride, sr, lane);
-    stage_out<P>(cur, L, lane);
+    if (PB == 0 && more)
+      tile_swap_rsrc16(L, span_rsrc(cur.out, cur.npoly * 2 * kN),
+                       span_rsrc(nxt.in, nxt.npoly * 2 * kN), lane);
+    else
+      stage_out<P>(cur, L, lane);
     wave_sync();
     cur = nxt;
   }

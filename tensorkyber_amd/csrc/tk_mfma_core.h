@@ -88,6 +88,41 @@ __device__ __forceinline__ void tile_out_rsrc(const int16_t *L, __amdgpu_buffer_
   }
 }
 
+// Write a 16-poly tile from the slab (rout) and load the next tile into it
+// (rin).  The next tile's loads go out before the stores, so waiting for
+// them, before their LDS writes, does not wait for these stores as well
+// (vector memory operations complete in issue order); the stores complete
+// during the next tile's passes.  Costs the 16 registers of half the
+// stores' data beyond staging in and out one after the other.
+__device__ __forceinline__ void tile_swap_rsrc16(int16_t *L, __amdgpu_buffer_rsrc_t rout,
+                                                 __amdgpu_buffer_rsrc_t rin, int lane)
+{
+  typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+  v4i n[8];
+#pragma unroll
+  for (int i = 0; i < 8; i++)
+    n[i] = __builtin_bit_cast(v4i, __builtin_amdgcn_raw_buffer_load_b128(rin, lane * 16, i * 1024,
+                                                                         kBufferNT));
+#pragma unroll
+  for (int h = 0; h < 8; h += 4) {
+    v4i o[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int q = (h + i) * 64 + lane, pp = q >> 5, ch = q & 31;
+      o[i] = *reinterpret_cast<const v4i *>(L + pp * kStride + ch * 8);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+      __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4u, o[i]), rout,
+                                             lane * 16 + (h + i) * 1024, 0, kBufferNT);
+  }
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    const int q = i * 64 + lane, pp = q >> 5, ch = q & 31;
+    *reinterpret_cast<v4i *>(L + pp * kStride + ch * 8) = n[i];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Packed small-coefficient input, the format of the reference's pack_small
 // (ntt_bench.cu:749-757): BITS-bit two's-complement fields, 8/BITS fields per
