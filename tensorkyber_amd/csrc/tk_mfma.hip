@@ -161,13 +161,13 @@ template <int BITS>
 struct PackedBits<PackedTiles<BITS>> : std::integral_constant<int, BITS> {
 };
 
-// At least three waves per SIMD (at most 168 VGPRs): without the bound the
-// scheduler trades a wave of occupancy for longer-lived staging registers.
-// The int16 kernels take 152 VGPRs (three waves: the tile swap holds the next
-// tile's loads and half the stores' data at once), the packed ones 128
-// (four).
-template <int FORM, bool INV, int WAVES, class Tiles>
-__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(3))) void k_mfma16(Tiles io,
+// Four waves per SIMD (at most 128 VGPRs), which LDS allows too (an
+// 8,448-byte slab per wave).  SWAP: int16 tiles swap (below); without it each tile is
+// staged in after the previous one is stored, which the launcher picks when
+// every wave has one tile: the swap's prologue then has nothing to overlap
+// (262,144 polys: 47-48 against 50-51 us forward).
+template <int FORM, bool INV, int WAVES, class Tiles, bool SWAP>
+__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(4))) void k_mfma16(Tiles io,
                                                         const DeviceTables *__restrict__ tab)
 {
   constexpr int P = 16;
@@ -215,21 +215,25 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(3)))
   // measured faster than loading after the stores: 180-182 against 183 us
   // per 2^20-poly int16 pass, 126-130 against 138-140 us for the 2-bit
   // packed forward.  vm_drain keeps the loop's waits from covering the
-  // prologue's loads.
+  // prologue's loads.  Without SWAP (one tile per wave) the int16 tile is
+  // staged in at the top of the loop and the prologue is not drained, so the
+  // matrices' loads stay in flight through the tile's first pass.
   const size_t ntiles = tile_count<P>(io);
   const size_t stride = (size_t)gridDim.x * WAVES;
   size_t tile = (size_t)blockIdx.x * WAVES + w;
   TileIo cur = tile_io<P>(io, tile < ntiles ? tile : 0);
   StageRegs<P, Tiles> sr;
+  constexpr bool kSwap = PB == 0 && SWAP;
   stage_load<P>(io, tile, sr, lane);
-  if constexpr (PB == 0)
+  if constexpr (kSwap)
     if (tile < ntiles)
       stage_in<P>(io, cur, L, sr, lane);
-  vm_drain();
+  if constexpr (PB != 0 || SWAP)
+    vm_drain();
   for (; tile < ntiles; tile += stride) {
     const bool more = tile + stride < ntiles;
     const TileIo nxt = tile_io<P>(io, more ? tile + stride : tile);
-    if constexpr (PB != 0)
+    if constexpr (!kSwap)
       stage_in<P>(io, cur, L, sr, lane);
     wave_sync();
     if (!INV) {
@@ -249,7 +253,7 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(3)))
       pass_m16<FORM>(Pp, g, a1lo, a1hi, bias1);
     }
     wave_sync();
-    if (PB == 0 && more)
+    if (kSwap && more)
       tile_swap_rsrc16(L, span_rsrc(cur.out, cur.npoly * 2 * kN),
                        span_rsrc(nxt.in, nxt.npoly * 2 * kN), lane);
     else
@@ -344,9 +348,19 @@ template <int SHAPE, int FORM, bool INV, class Tiles>
 int launch_one(int waves, int ncu, int wgs, const Tiles &io, const DeviceTables *tab,
                hipStream_t s)
 {
-  if (SHAPE == 16)
-    return waves == 2 ? launch_persistent(k_mfma16<FORM, INV, 2, Tiles>, 2, 16, ncu, wgs, io, tab, s)
-                      : launch_persistent(k_mfma16<FORM, INV, 4, Tiles>, 4, 16, ncu, wgs, io, tab, s);
+  if (SHAPE == 16) {
+    // The swap pays when a wave has more than one tile: the grid (capped at
+    // `wgs` times the resident workgroups) then has fewer wave slots than
+    // tiles.  Both instances have the same residency.
+    auto pick = [&](auto swapped, auto plain, int nw) {
+      const int per_cu = resident_per_cu((const void *)swapped, 64 * nw) * (wgs > 0 ? wgs : 1);
+      const bool swap = host_tile_count(io, 16) > (size_t)ncu * per_cu * nw;
+      return swap ? launch_persistent(swapped, nw, 16, ncu, wgs, io, tab, s)
+                  : launch_persistent(plain, nw, 16, ncu, wgs, io, tab, s);
+    };
+    return waves == 2 ? pick(k_mfma16<FORM, INV, 2, Tiles, true>, k_mfma16<FORM, INV, 2, Tiles, false>, 2)
+                      : pick(k_mfma16<FORM, INV, 4, Tiles, true>, k_mfma16<FORM, INV, 4, Tiles, false>, 4);
+  }
   return waves == 2 ? launch_persistent(k_mfma32<FORM, INV, 2, Tiles>, 2, 32, ncu, wgs, io, tab, s)
                     : launch_persistent(k_mfma32<FORM, INV, 4, Tiles>, 4, 32, ncu, wgs, io, tab, s);
 }
@@ -394,10 +408,10 @@ void prepare_ragged_launch(int shape, int waves)
     one(waves == 2 ? (const void *)k_mfma32<kFormCentred, true, 2, RaggedTiles>
                    : (const void *)k_mfma32<kFormCentred, true, 4, RaggedTiles>);
   } else {
-    one(waves == 2 ? (const void *)k_mfma16<kFormCanon, false, 2, RaggedTiles>
-                   : (const void *)k_mfma16<kFormCanon, false, 4, RaggedTiles>);
-    one(waves == 2 ? (const void *)k_mfma16<kFormCentred, true, 2, RaggedTiles>
-                   : (const void *)k_mfma16<kFormCentred, true, 4, RaggedTiles>);
+    one(waves == 2 ? (const void *)k_mfma16<kFormCanon, false, 2, RaggedTiles, true>
+                   : (const void *)k_mfma16<kFormCanon, false, 4, RaggedTiles, true>);
+    one(waves == 2 ? (const void *)k_mfma16<kFormCentred, true, 2, RaggedTiles, true>
+                   : (const void *)k_mfma16<kFormCentred, true, 4, RaggedTiles, true>);
   }
 }
 

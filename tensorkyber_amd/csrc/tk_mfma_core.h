@@ -92,8 +92,9 @@ __device__ __forceinline__ void tile_out_rsrc(const int16_t *L, __amdgpu_buffer_
 // (rin).  The next tile's loads go out before the stores, so waiting for
 // them, before their LDS writes, does not wait for these stores as well
 // (vector memory operations complete in issue order); the stores complete
-// during the next tile's passes.  Costs the 16 registers of half the
-// stores' data beyond staging in and out one after the other.
+// during the next tile's passes.  The stores' data goes through registers
+// two chunks at a time (a scheduling barrier per pair), so the swap holds 8
+// registers beyond the loads' 32 and the kernel keeps 128 VGPRs.
 __device__ __forceinline__ void tile_swap_rsrc16(int16_t *L, __amdgpu_buffer_rsrc_t rout,
                                                  __amdgpu_buffer_rsrc_t rin, int lane)
 {
@@ -104,17 +105,18 @@ __device__ __forceinline__ void tile_swap_rsrc16(int16_t *L, __amdgpu_buffer_rsr
     n[i] = __builtin_bit_cast(v4i, __builtin_amdgcn_raw_buffer_load_b128(rin, lane * 16, i * 1024,
                                                                          kBufferNT));
 #pragma unroll
-  for (int h = 0; h < 8; h += 4) {
-    v4i o[4];
+  for (int h = 0; h < 8; h += 2) {
+    v4i o[2];
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
+    for (int i = 0; i < 2; i++) {
       const int q = (h + i) * 64 + lane, pp = q >> 5, ch = q & 31;
       o[i] = *reinterpret_cast<const v4i *>(L + pp * kStride + ch * 8);
     }
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+    for (int i = 0; i < 2; i++)
       __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4u, o[i]), rout,
                                              lane * 16 + (h + i) * 1024, 0, kBufferNT);
+    __builtin_amdgcn_sched_barrier(0);
   }
 #pragma unroll
   for (int i = 0; i < 8; i++) {
