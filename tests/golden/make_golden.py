@@ -14,7 +14,7 @@ with glibc rand(), ntt_bench.cu:941-961), so these are produced here:
   outputs of the full-size benchmark configurations, inputs from the
   project's SplitMix64 streams (include/tkyber_workload.h).
 
-Usage: python tests/golden/make_golden.py
+Usage: python tests/golden/make_golden.py [--digests-only | --ranks-only]
 """
 from __future__ import annotations
 
@@ -145,11 +145,36 @@ def digests(ref: Reference) -> dict:
     print(f"config 5: {d['config5']}", flush=True)
     d["packed"] = packed_digests(ref)
     d["kpke"] = kpke_digests(ref)
+    d.update(rank_digests(ref))
+    return d
+
+
+def rank_digests(ref: Reference) -> dict:
+    """Configuration 5's product and the K-PKE cores at ranks 2 and 4, on the
+    same 65,536-instance workloads as rank 3."""
+    d = {"config5_by_rank": {}, "kpke_by_rank": {}}
+    for k in (2, 4):
+        A, s = matvec_inputs(k=k)
+        t = ref.matvec(A, s, k)
+        d["config5_by_rank"][str(k)] = {"ninst": int(s.shape[0]), "k": k,
+                                        "A": tk.canon_digest(A), "s": tk.canon_digest(s),
+                                        "t": tk.canon_digest(t)}
+        print(f"config 5, k = {k}: {d['config5_by_rank'][str(k)]}", flush=True)
+        d["kpke_by_rank"][str(k)] = kpke_digests(ref, k=k)
     return d
 
 
 def main():
     ref = Reference()
+    path = os.path.join(HERE, "digests.json")
+    if "--ranks-only" in sys.argv:  # add the rank 2 and 4 digests to the existing file
+        with open(path) as f:
+            d = json.load(f)
+        d.update(rank_digests(ref))
+        with open(path, "w") as f:
+            json.dump(d, f, indent=1, sort_keys=True)
+        print("wrote digests.json")
+        return
     if "--digests-only" not in sys.argv:
         fx = fixtures(ref)
         np.savez_compressed(os.path.join(HERE, "fixtures.npz"), **fx)

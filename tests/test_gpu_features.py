@@ -174,12 +174,16 @@ def test_matvec_in_place_over_s(tk, gpu, oracle):
     assert np.array_equal(ds.cpu().numpy().reshape(ninst, k, N), centred(oracle.matvec(A, s, k)))
 
 
-def test_matvec_full_size_digest(tk, gpu, digests):
-    """Configuration 5: 65,536 instances, k = 3."""
+@pytest.mark.parametrize("k", [2, 3, 4])
+def test_matvec_full_size_digest(tk, gpu, digests, k):
+    """Configuration 5's product, 65,536 instances, at rank k (the workload of
+    configuration 5 itself at k = 3)."""
     from golden.make_golden import matvec_inputs
-    A, s = matvec_inputs()
-    t = tk.matvec(_dev(A, gpu), _dev(s, gpu), 3).cpu().numpy()
-    assert tk.canon_digest(t) == digests["config5"]["t"]
+    d = digests["config5"] if k == 3 else digests["config5_by_rank"][str(k)]
+    A, s = matvec_inputs(k=k)
+    assert (tk.canon_digest(A), tk.canon_digest(s)) == (d["A"], d["s"])
+    t = tk.matvec(_dev(A, gpu), _dev(s, gpu), k).cpu().numpy()
+    assert tk.canon_digest(t) == d["t"]
 
 
 @pytest.mark.parametrize("cfg", [1, 2, 3])

@@ -103,11 +103,12 @@ def test_kpke_message_round_trip(tk, gpu, k):
     assert np.array_equal(((dec * 2 + Q // 2) // Q) & 1, w["bits"])
 
 
-def test_kpke_full_size_digests(tk, gpu, digests):
-    """65,536 instances of rank 3: every output centred and equal, mod q, to
+@pytest.mark.parametrize("k", [2, 3, 4])
+def test_kpke_full_size_digests(tk, gpu, digests, k):
+    """65,536 instances of rank k: every output centred and equal, mod q, to
     the compiled reference's (so raw-equal)."""
-    d = digests["kpke"]
-    k = d["k"]
+    d = digests["kpke"] if k == 3 else digests["kpke_by_rank"][str(k)]
+    assert d["k"] == k
     w = tk.kpke_workload(d["ninst"], k)
     t_hat, s_hat = tk.kpke_keygen(_dev(w["A_hat"], gpu), _dev(w["s"], gpu), _dev(w["e"], gpu), k)
     u, v = tk.kpke_encrypt(_dev(_transpose(w["A_hat"]), gpu), t_hat, _dev(w["r"], gpu),

@@ -95,10 +95,11 @@ def test_kpke_message_round_trip(oracle, tk, k):
     assert np.array_equal(decoded, bits)
 
 
-def test_kpke_full_size_digests(oracle, tk, digests):
-    """65,536 instances of rank 3 against the compiled reference's digests."""
-    d = digests["kpke"]
-    k = d["k"]
+@pytest.mark.parametrize("k", [2, 3, 4])
+def test_kpke_full_size_digests(oracle, tk, digests, k):
+    """65,536 instances of rank k against the compiled reference's digests."""
+    d = digests["kpke"] if k == 3 else digests["kpke_by_rank"][str(k)]
+    assert d["k"] == k
     w = tk.kpke_workload(d["ninst"], k)
     assert tk.canon_digest(w["A_hat"]) == d["A_hat"]
     t_hat, s_hat = oracle.kpke_keygen(w["A_hat"], w["s"], w["e"], k)

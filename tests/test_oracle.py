@@ -164,3 +164,8 @@ def test_ragged_and_matvec_digests(oracle, tk, digests):
     dm = digests["config5"]
     assert (tk.canon_digest(A), tk.canon_digest(s)) == (dm["A"], dm["s"])
     assert tk.canon_digest(oracle.matvec(A, s, 3)) == dm["t"]
+    for k in (2, 4):  # the same product at the other ranks
+        A, s = matvec_inputs(k=k)
+        dm = digests["config5_by_rank"][str(k)]
+        assert (tk.canon_digest(A), tk.canon_digest(s)) == (dm["A"], dm["s"])
+        assert tk.canon_digest(oracle.matvec(A, s, k)) == dm["t"]
