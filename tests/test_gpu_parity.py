@@ -155,3 +155,24 @@ def test_default_entry_points(tk, gpu, oracle):
     assert tk.lib().basemul_batch(r.data_ptr(), d.data_ptr(), d.data_ptr(), npoly, s) == 0
     h = _host(d)
     assert np.array_equal(_host(r), oracle.basemul(h, h))
+
+
+@pytest.mark.parametrize("npoly", [32768 * 16 + 1, 32768 * 24 + 7, (1 << 20) - 3])
+def test_mfma16_tile_swap_sizes(tk, gpu, oracle, npoly):
+    """Batches around the size where the MFMA16 launcher switches to the tile
+    swap (more tiles than the grid's wave slots: 256 CUs x 4 workgroups x 8 x
+    4 waves = 32,768 tiles on MI355X), each ending in a partial tile: some
+    waves swap into a second tile and others do not, in place and out of
+    place, against the oracle on every poly."""
+    import torch
+    print(f"mfma16 tile swap npoly={npoly}", flush=True)
+    x = _inputs(tk, "edge", npoly, 70 + npoly % 97)
+    src = _dev(x, gpu)
+    fwd = tk.ntt(src, method="mfma16", form="canonical")
+    want_f = canon(oracle.ntt(x))
+    assert np.array_equal(_host(fwd), want_f), "forward"
+    inv = fwd.clone()
+    tk.invntt(inv, out=inv, method="mfma16", form="centered")
+    assert np.array_equal(_host(inv), centred(oracle.invntt(want_f))), "inverse in place"
+    assert np.array_equal(_host(src), x), "input changed"
+    torch.cuda.synchronize()
