@@ -978,15 +978,14 @@ __attribute__((amdgpu_waves_per_eu(TABLE ? 3 : 4))) void k_decrypt(
 constexpr int kFusedOversub = 4;
 
 int launch_fused(int k, int mode, int s_bits, const FusedIo &io, const DeviceTables *tab, int ncu,
-                 hipStream_t stream)
+                 const LaunchSink &stream)
 {
   const size_t ntiles = (io.ninst + kTileInst - 1) / kTileInst;
   auto go = [&](auto kernel, int waves) {
     const size_t cap =
         (size_t)ncu * resident_per_cu((const void *)kernel, 64 * waves) * kFusedOversub;
     const dim3 grid((unsigned)(ntiles < cap ? ntiles : cap));
-    hipLaunchKernelGGL(kernel, grid, dim3(64 * waves), 0, stream, io, tab);
-    return 0;
+    return sink_launch(stream, kernel, grid, dim3(64 * waves), io, tab);
   };
   auto with_k = [&](auto mode_c, auto bits_c, auto table_c) {
     constexpr int M = decltype(mode_c)::value, B = decltype(bits_c)::value;
@@ -1010,8 +1009,7 @@ int launch_fused(int k, int mode, int s_bits, const FusedIo &io, const DeviceTab
   // Wide matvec: one workgroup per CU (its LDS), persistent over the tiles.
   auto wide = [&](auto kernel, int waves) {
     const size_t grid = ntiles < (size_t)ncu ? ntiles : (size_t)ncu;
-    hipLaunchKernelGGL(kernel, dim3((unsigned)grid), dim3(64 * waves), 0, stream, io, tab);
-    return 0;
+    return sink_launch(stream, kernel, dim3((unsigned)grid), dim3(64 * waves), io, tab);
   };
   switch (mode) {
     case kFusedMatvec: {
@@ -1044,9 +1042,8 @@ int launch_fused(int k, int mode, int s_bits, const FusedIo &io, const DeviceTab
         const size_t cap =
             (size_t)ncu * resident_per_cu((const void *)kernel, 64 * waves) * oversub;
         const size_t wgs = waves == kDecryptWideWaves ? ntiles : (ntiles + waves - 1) / waves;
-        hipLaunchKernelGGL(kernel, dim3((unsigned)(wgs < cap ? wgs : cap)), dim3(64 * waves), 0,
-                           stream, io, tab);
-        return 0;
+        return sink_launch(stream, kernel, dim3((unsigned)(wgs < cap ? wgs : cap)),
+                           dim3(64 * waves), io, tab);
       };
       constexpr int kW = kDecryptWideWaves, kT = kDecryptWaves;
       switch (k * 2 + (io.inst ? 1 : 0)) {

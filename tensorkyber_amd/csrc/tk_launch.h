@@ -1,6 +1,7 @@
 // Host-side launchers of the kernels (one per .hip file); called by the C ABI
-// in tk_api.hip after argument validation.  Each returns 0 or -1 for an
-// unsupported form; HIP launch errors are read by the caller.
+// in tk_api.hip after argument validation.  Each returns 0, -1 for an
+// unsupported form, or TK_EDEVICE when a graph node could not be added;
+// HIP launch errors are read by the caller.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -8,6 +9,47 @@
 #include "tk_internal.h"
 
 namespace tk {
+
+int record_hip(hipError_t e);
+
+// Where a launcher puts its kernel: launched on a stream, or added to a
+// graph under construction as a kernel node with no dependencies.  Plans
+// build their graphs this way rather than by stream capture: a capture is a
+// window in which what other host threads do on the device can invalidate
+// it or be refused.
+struct LaunchSink {
+  hipStream_t stream = nullptr;
+  hipGraph_t graph = nullptr;
+  LaunchSink(hipStream_t s) : stream(s) {}  // NOLINT: a stream is a sink
+  explicit LaunchSink(hipGraph_t g) : graph(g) {}
+};
+
+template <class T>
+struct NoDeduce {
+  using type = T;
+};
+
+// kernel<<<grid, block>>>(args...) into the sink: 0, or TK_EDEVICE if the
+// graph node could not be added (launch errors are read by the caller).
+template <class... P>
+int sink_launch(const LaunchSink &sk, void (*kernel)(P...), dim3 grid, dim3 block,
+                typename NoDeduce<P>::type... args)
+{
+  if (!sk.graph) {
+    hipLaunchKernelGGL(kernel, grid, block, 0, sk.stream, args...);
+    return 0;
+  }
+  void *params[] = {static_cast<void *>(&args)...};
+  hipKernelNodeParams kp{};
+  kp.func = reinterpret_cast<void *>(kernel);
+  kp.gridDim = grid;
+  kp.blockDim = block;
+  kp.sharedMemBytes = 0;
+  kp.kernelParams = params;
+  kp.extra = nullptr;
+  hipGraphNode_t node = nullptr;
+  return record_hip(hipGraphAddKernelNode(&node, sk.graph, nullptr, 0, &kp));
+}
 
 // A contiguous batch: poly p at in + p*256 (read) and out + p*256 (written).
 struct LinearTiles {
@@ -82,7 +124,7 @@ void prepare_ragged_launch(int shape, int waves);
 
 // io.tiles must have been cut for shape's tile height (16 or 32 polys).
 int launch_mfma_ragged(int shape, bool inverse, int form, const RaggedTiles &io,
-                       const DeviceTables *tab, const LaunchShape &ls, hipStream_t stream);
+                       const DeviceTables *tab, const LaunchShape &ls, const LaunchSink &sink);
 
 // Forward transform of packed small-coefficient polys (bits 2, 3 or 4).
 int launch_mfma_packed(int shape, int bits, int form, int16_t *out, const uint8_t *in,
@@ -136,7 +178,7 @@ struct FusedIo {
 
 // s_bits = 0: x is int16 polys; 2, 3 or 4: x is packed (kFusedMatvec only).
 int launch_fused(int k, int mode, int s_bits, const FusedIo &io, const DeviceTables *tab, int ncu,
-                 hipStream_t stream);
+                 const LaunchSink &sink);
 
 int launch_basemul(int form, int16_t *r, const int16_t *a, const int16_t *b, size_t npoly,
                    const DeviceTables *tab, int ncu, hipStream_t stream);

@@ -332,7 +332,7 @@ inline size_t host_tile_count(const PackedTiles<BITS> &io, int P)
 
 template <class K, class Tiles>
 int launch_persistent(K kernel, int waves, int P, int ncu, int oversub, const Tiles &io,
-                      const DeviceTables *tab, hipStream_t s)
+                      const DeviceTables *tab, const LaunchSink &s)
 {
   const int block = 64 * waves;
   const int per_cu = resident_per_cu((const void *)kernel, block) * (oversub > 0 ? oversub : 1);
@@ -340,13 +340,12 @@ int launch_persistent(K kernel, int waves, int P, int ncu, int oversub, const Ti
   const size_t need = (ntiles + waves - 1) / waves;
   const size_t cap = (size_t)ncu * per_cu;
   const dim3 grid((unsigned)(need < cap ? need : cap));
-  hipLaunchKernelGGL(kernel, grid, dim3(block), 0, s, io, tab);
-  return 0;
+  return sink_launch(s, kernel, grid, dim3(block), io, tab);
 }
 
 template <int SHAPE, int FORM, bool INV, class Tiles>
 int launch_one(int waves, int ncu, int wgs, const Tiles &io, const DeviceTables *tab,
-               hipStream_t s)
+               const LaunchSink &s)
 {
   if (SHAPE == 16) {
     // The swap pays when a wave has more than one tile: the grid (capped at
@@ -367,7 +366,7 @@ int launch_one(int waves, int ncu, int wgs, const Tiles &io, const DeviceTables 
 
 template <int SHAPE, bool INV, class Tiles>
 int launch_form(int form, int waves, int ncu, int wgs, const Tiles &io, const DeviceTables *tab,
-                hipStream_t s)
+                const LaunchSink &s)
 {
   switch (form) {
     case kFormCanon: return launch_one<SHAPE, kFormCanon, INV>(waves, ncu, wgs, io, tab, s);
@@ -378,7 +377,7 @@ int launch_form(int form, int waves, int ncu, int wgs, const Tiles &io, const De
 
 template <class Tiles>
 int launch_any(int shape, bool inverse, int form, const Tiles &io, const DeviceTables *tab,
-               const LaunchShape &ls, hipStream_t stream)
+               const LaunchShape &ls, const LaunchSink &stream)
 {
   if (ls.waves != 2 && ls.waves != 4)
     return -1;
@@ -416,7 +415,7 @@ void prepare_ragged_launch(int shape, int waves)
 }
 
 int launch_mfma_ragged(int shape, bool inverse, int form, const RaggedTiles &io,
-                       const DeviceTables *tab, const LaunchShape &ls, hipStream_t stream)
+                       const DeviceTables *tab, const LaunchShape &ls, const LaunchSink &stream)
 {
   return launch_any(shape, inverse, form, io, tab, ls, stream);
 }

@@ -9,8 +9,8 @@
 // polys exactly as it stages a linear batch.  The autotuner picks the tile
 // shape, waves per workgroup and grid from timings of the plan's own
 // requests.  Executing the plan is one dispatch of that kernel whatever the
-// number of requests, with no host work; the launch is also recorded into a
-// hipGraph, which callers can embed in their own graphs.
+// number of requests, with no host work; the launch is also the one kernel
+// node of a hipGraph, which callers can embed in their own graphs.
 #include <hip/hip_runtime.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -273,12 +273,14 @@ void plan_free(tk_ragged_plan *p)
   delete p;
 }
 
-int plan_launch(tk_ragged_plan *p, hipStream_t s)
+// The plan's kernel on a stream, or as the node of its graph.
+int plan_launch(tk_ragged_plan *p, const LaunchSink &s)
 {
   const int form = p->direction ? kFormCentred : kFormCanon;
-  if (launch_mfma_ragged(p->shape, p->direction != 0, form, p->tiles, p->tab,
-                         LaunchShape{p->ncu, p->waves, p->oversub}, s))
-    return TK_ENOTSUP;
+  const int rc = launch_mfma_ragged(p->shape, p->direction != 0, form, p->tiles, p->tab,
+                                    LaunchShape{p->ncu, p->waves, p->oversub}, s);
+  if (rc)
+    return rc == TK_EDEVICE ? rc : TK_ENOTSUP;
   return after_launch();
 }
 
@@ -397,19 +399,12 @@ int tk_ragged_plan_create(tk_ragged_plan **plan, const tk_request *table, size_t
   }
   p->tiles = RaggedTiles{static_cast<const TileRef *>(p->dmem), tiles.size()};
 
-  // Record the launch into a graph on a private capture stream.
-  hipStream_t cap = nullptr;
-  if ((rc = record_hip(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking)))) {
-    plan_free(p);
-    return rc;
-  }
-  rc = record_hip(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
-  if (rc == TK_OK) {
-    const int lrc = plan_launch(p, cap);
-    const int erc = record_hip(hipStreamEndCapture(cap, &p->graph));
-    rc = lrc ? lrc : erc;
-  }
-  (void)hipStreamDestroy(cap);
+  // The launch as the one kernel node of the plan's graph, built node by
+  // node (no stream capture, so other host threads' device work cannot
+  // interfere with it).
+  rc = record_hip(hipGraphCreate(&p->graph, 0));
+  if (rc == TK_OK)
+    rc = plan_launch(p, LaunchSink(p->graph));
   if (rc) {
     plan_free(p);
     return rc;

@@ -6,9 +6,9 @@
 //   * groups the instances by (operation, rank): one fused kernel per group,
 //     whatever the number of requests, each instance found through a device
 //     table of its eight buffer pointers;
-//   * orders the groups by work, largest first, and records them as parallel
-//     branches of one hipGraph (fork/join through events), so small groups
-//     fill the CUs the large ones leave idle at their tails.
+//   * orders the groups by work, largest first, and builds them as parallel
+//     branches of one hipGraph (one kernel node each, no dependencies), so
+//     small groups fill the CUs the large ones leave idle at their tails.
 // Executing the plan replays the graph: one launch call per batch.
 #include <hip/hip_runtime.h>
 
@@ -98,60 +98,27 @@ void plan_free(tk_fused_plan *p)
   delete p;
 }
 
-int launch_group(const tk_fused_plan *p, const Group &g, hipStream_t s)
+int launch_group(const tk_fused_plan *p, const Group &g, const tk::LaunchSink &s)
 {
   tk::FusedIo io{};
   io.ninst = g.ninst;
   io.inst = p->table + g.first;
-  if (tk::launch_fused(g.k, g.mode, 0, io, p->tab, p->ncu, s))
-    return TK_ENOTSUP;
+  const int rc = tk::launch_fused(g.k, g.mode, 0, io, p->tab, p->ncu, s);
+  if (rc)
+    return rc == TK_EDEVICE ? rc : TK_ENOTSUP;
   return tk::after_launch();
 }
 
-// The groups as parallel branches of a graph captured on a private stream.
-int capture(tk_fused_plan *p)
+// The groups as parallel branches of one graph: one kernel node each, with
+// no dependencies between them, built node by node (no stream capture, so
+// other host threads' device work cannot interfere with it).
+int build_graph(tk_fused_plan *p)
 {
-  const size_t ng = p->groups.size();
-  std::vector<hipStream_t> br(ng, nullptr);
-  std::vector<hipEvent_t> join(ng, nullptr);
-  hipStream_t cap = nullptr;
-  hipEvent_t fork = nullptr;
-  int rc = tk::record_hip(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
-  if (rc == TK_OK)
-    rc = tk::record_hip(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
-  for (size_t g = 0; g < ng && rc == TK_OK; g++) {
-    rc = tk::record_hip(hipStreamCreateWithFlags(&br[g], hipStreamNonBlocking));
-    if (rc == TK_OK)
-      rc = tk::record_hip(hipEventCreateWithFlags(&join[g], hipEventDisableTiming));
-  }
-  if (rc == TK_OK)
-    rc = tk::record_hip(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
-  if (rc == TK_OK) {
-    int lrc = tk::record_hip(hipEventRecord(fork, cap));
-    for (size_t g = 0; g < ng && lrc == TK_OK; g++) {
-      lrc = tk::record_hip(hipStreamWaitEvent(br[g], fork, 0));
-      if (lrc == TK_OK)
-        lrc = launch_group(p, p->groups[g], br[g]);
-      if (lrc == TK_OK)
-        lrc = tk::record_hip(hipEventRecord(join[g], br[g]));
-      if (lrc == TK_OK)
-        lrc = tk::record_hip(hipStreamWaitEvent(cap, join[g], 0));
-    }
-    const int erc = tk::record_hip(hipStreamEndCapture(cap, &p->graph));
-    rc = lrc ? lrc : erc;
-  }
+  int rc = tk::record_hip(hipGraphCreate(&p->graph, 0));
+  for (size_t g = 0; g < p->groups.size() && rc == TK_OK; g++)
+    rc = launch_group(p, p->groups[g], tk::LaunchSink(p->graph));
   if (rc == TK_OK)
     rc = tk::record_hip(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
-  for (size_t g = 0; g < ng; g++) {
-    if (join[g])
-      (void)hipEventDestroy(join[g]);
-    if (br[g])
-      (void)hipStreamDestroy(br[g]);
-  }
-  if (fork)
-    (void)hipEventDestroy(fork);
-  if (cap)
-    (void)hipStreamDestroy(cap);
   return rc;
 }
 
@@ -239,7 +206,7 @@ int tk_fused_plan_create(tk_fused_plan **out, const tk_fused_request *reqs, size
   }
   if ((rc = tk::record_hip(
            hipMemcpy(p->table, host.data(), total * sizeof(tk::FusedInst), hipMemcpyHostToDevice))) ||
-      (rc = capture(p))) {
+      (rc = build_graph(p))) {
     plan_free(p);
     return rc;
   }

@@ -121,7 +121,8 @@ def test_concurrent_host_threads(tk, gpu, oracle):
                 sp = ctypes.c_void_p(stream.cuda_stream)
                 rc = tk.lib().tk_ragged_plan_create(ctypes.byref(plan), ctypes.addressof(table),
                                                     len(reqs), 0, tk.METHOD_MFMA16, sp)
-                assert rc == 0, f"thread {tid}: plan create {rc}"
+                assert rc == 0, \
+                    f"thread {tid}: plan create {rc} (HIP error {tk.lib().tk_last_hip_error()})"
                 barrier.wait()
                 for rep in range(reps):
                     out = torch.empty_like(dx)
