@@ -727,9 +727,9 @@ def load_traffic(kernel, npoly):
 # Kernel name -> bench kernel label (forward canonical, inverse centred, as
 # ntt_batch / invntt_batch launch them).
 # The 2^20-poly batch gives each MFMA16 wave two tiles: the tile-swap
-# instance (last template argument true) runs.
-KERNEL_LABELS = (("k_mfma16<1, false, 4, tk::LinearTiles, true>", "mfma16_fwd"),
-                 ("k_mfma16<2, true, 4, tk::LinearTiles, true>", "mfma16_inv"),
+# instance (fifth template argument true) runs.
+KERNEL_LABELS = (("k_mfma16<1, false, 4, tk::LinearTiles, true", "mfma16_fwd"),
+                 ("k_mfma16<2, true, 4, tk::LinearTiles, true", "mfma16_inv"),
                  ("k_mfma32<1, false, 4, tk::LinearTiles>", "mfma32_fwd"),
                  ("k_mfma32<2, true, 4, tk::LinearTiles>", "mfma32_inv"),
                  ("k_ntt_butterfly", "butterfly_fwd"), ("k_invntt_butterfly", "butterfly_inv"))

@@ -94,11 +94,13 @@ struct PackedTiles {
 // 181-183 us per pass, against 187-188 at 4 and 189-191 at 16) and 12 for
 // 32-poly tiles (180-183 / 184-188 us forward / inverse, against 188-190 /
 // 190-192 at 8; the inverse slows again from 16).  Packed batches load few
-// bytes per tile and use 1, with workgroups of 2 waves for 2-bit input and
-// 4 for 3 and 4 bits: 2^20 polys take 126-127 us at 2 bits (128-129 with 4
-// waves, 129-130 at 2x) and 135-136 us at 3 bits (137-138 with 2 waves);
-// 4x took 141-143 us at 2 bits.  Ragged plans let the autotuner choose,
-// starting from the linear shape.
+// bytes per tile: 32-poly tiles use 1, with workgroups of 2 waves for 2-bit
+// input and 4 for 3 and 4 bits; 16-poly tiles run one 16-wave workgroup per
+// CU whose waves take tiles from a queue in LDS (k_mfma16, QUEUED): 2^20
+// polys take 119-120 us at 2 bits and 131-132 us at 3 or 4, where the
+// 1x grid of 2- or 4-wave workgroups striding over the tiles took 126 and
+// 135 us (and 4x 141-143 us at 2 bits).  Ragged plans let the autotuner
+// choose, starting from the linear shape.
 constexpr int kLinearOversub = 8;
 constexpr int kLinearOversub32 = 12;
 constexpr int kPackedOversub = 1;

@@ -166,13 +166,25 @@ struct PackedBits<PackedTiles<BITS>> : std::integral_constant<int, BITS> {
 // staged in after the previous one is stored, which the launcher picks when
 // every wave has one tile: the swap's prologue then has nothing to overlap
 // (262,144 polys: 47-48 against 50-51 us forward).
-template <int FORM, bool INV, int WAVES, class Tiles, bool SWAP>
+//
+// QUEUED: the waves of a workgroup take their tiles from a counter in LDS
+// instead of striding over them: workgroup b's k-th tile is k * grid + b,
+// handed to whichever of its waves asks next.  With one 16-wave workgroup
+// per CU (the launcher's shape for packed input) the four waves of a SIMD
+// then share its tiles.  Striding, each wave had the same number of tiles
+// while the SIMD issues for the oldest ready wave first: the four finished
+// in a staircase (2^20 2-bit polys: at about 60, 80, 100 and 118 us), and
+// the last third of the kernel ran on fewer waves than the SIMD holds.
+template <int FORM, bool INV, int WAVES, class Tiles, bool SWAP, bool QUEUED = false>
 __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(4))) void k_mfma16(Tiles io,
                                                         const DeviceTables *__restrict__ tab)
 {
   constexpr int P = 16;
   __shared__ __align__(16) int16_t s_poly[WAVES][P * kStride];
   __shared__ __align__(16) int32_t s_bias[16][16];
+  __shared__ uint32_t s_next;  // QUEUED: the workgroup's next tile number
+  if (QUEUED && threadIdx.x == 0)
+    s_next = 0;
   const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), g = lane >> 4, t = lane & 15;
 
   const int32_t(*bias_c)[16] = INV ? tab->bias_inv2 : tab->bias_fwd2;
@@ -220,7 +232,13 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(4)))
   // matrices' loads stay in flight through the tile's first pass.
   const size_t ntiles = tile_count<P>(io);
   const size_t stride = (size_t)gridDim.x * WAVES;
-  size_t tile = (size_t)blockIdx.x * WAVES + w;
+  auto take = [&]() -> size_t {
+    uint32_t k = 0;
+    if (lane == 0)
+      k = __hip_atomic_fetch_add(&s_next, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    return (size_t)__builtin_amdgcn_readfirstlane(k) * gridDim.x + blockIdx.x;
+  };
+  size_t tile = QUEUED ? take() : (size_t)blockIdx.x * WAVES + w;
   TileIo cur = tile_io<P>(io, tile < ntiles ? tile : 0);
   StageRegs<P, Tiles> sr;
   constexpr bool kSwap = PB == 0 && SWAP;
@@ -230,16 +248,17 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(4)))
       stage_in<P>(io, cur, L, sr, lane);
   if constexpr (PB != 0 || SWAP)
     vm_drain();
-  for (; tile < ntiles; tile += stride) {
-    const bool more = tile + stride < ntiles;
-    const TileIo nxt = tile_io<P>(io, more ? tile + stride : tile);
+  for (size_t next; tile < ntiles; tile = next) {
+    next = QUEUED ? take() : tile + stride;
+    const bool more = next < ntiles;
+    const TileIo nxt = tile_io<P>(io, more ? next : tile);
     if constexpr (!kSwap)
       stage_in<P>(io, cur, L, sr, lane);
     wave_sync();
     if (!INV) {
       if constexpr (PB != 0) {
         pass_m16_packed<PB>(Pp, g, sr.v, a1lo, a1hi, bias1);
-        stage_load<P>(io, tile + stride, sr, lane);
+        stage_load<P>(io, next, sr, lane);
       } else {
         pass_m16<kFormInter>(Pp, g, a1lo, a1hi, bias1);
       }
@@ -322,6 +341,8 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2)))
   }
 }
 
+constexpr int kQueuedWaves = 16;  // one workgroup per CU, four waves per SIMD
+
 inline size_t host_tile_count(const LinearTiles &io, int P) { return (io.npoly + P - 1) / P; }
 inline size_t host_tile_count(const RaggedTiles &io, int) { return io.ntiles; }
 template <int BITS>
@@ -357,8 +378,18 @@ int launch_one(int waves, int ncu, int wgs, const Tiles &io, const DeviceTables 
       return swap ? launch_persistent(swapped, nw, 16, ncu, wgs, io, tab, s)
                   : launch_persistent(plain, nw, 16, ncu, wgs, io, tab, s);
     };
-    return waves == 2 ? pick(k_mfma16<FORM, INV, 2, Tiles, true>, k_mfma16<FORM, INV, 2, Tiles, false>, 2)
-                      : pick(k_mfma16<FORM, INV, 4, Tiles, true>, k_mfma16<FORM, INV, 4, Tiles, false>, 4);
+    // Packed input: one 16-wave workgroup per CU (its LDS), the waves
+    // taking tiles from the workgroup's queue (k_mfma16, QUEUED).
+    if constexpr (PackedBits<Tiles>::value != 0) {
+      const size_t ntiles = host_tile_count(io, 16);
+      const dim3 grid((unsigned)(ntiles < (size_t)ncu ? ntiles : (size_t)ncu));
+      return sink_launch(s, k_mfma16<FORM, INV, kQueuedWaves, Tiles, true, true>, grid,
+                         dim3(64 * kQueuedWaves), io, tab);
+    } else {
+      return waves == 2
+                 ? pick(k_mfma16<FORM, INV, 2, Tiles, true>, k_mfma16<FORM, INV, 2, Tiles, false>, 2)
+                 : pick(k_mfma16<FORM, INV, 4, Tiles, true>, k_mfma16<FORM, INV, 4, Tiles, false>, 4);
+    }
   }
   return waves == 2 ? launch_persistent(k_mfma32<FORM, INV, 2, Tiles>, 2, 32, ncu, wgs, io, tab, s)
                     : launch_persistent(k_mfma32<FORM, INV, 4, Tiles>, 4, 32, ncu, wgs, io, tab, s);

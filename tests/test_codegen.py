@@ -64,7 +64,7 @@ def _resources(remarks):
 
 
 def test_no_spills_and_tile_kernel_occupancy(compiled):
-    seen = four = 0
+    seen = four = queued = 0
     for src, _, remarks in compiled:
         for name, r in _resources(remarks).items():
             seen += 1
@@ -77,10 +77,14 @@ def test_no_spills_and_tile_kernel_occupancy(compiled):
             if "k_mfma16" in name or wide:
                 assert r["Occupancy [waves/SIMD]"] >= 4, f"{name}: occupancy below 4"
                 four += 1
+                # the queued packed instances: 16 waves per workgroup, tiles from LDS
+                # (k_mfma16<FORM, INV, 16, PackedTiles<B>, true, true>)
+                queued += "ELi16ENS_11PackedTiles" in name
             if "k_mfma32" in name:
                 assert r["Occupancy [waves/SIMD]"] >= 2, f"{name}: occupancy below 2"
     assert seen > 50, "kernel remarks not found"
-    assert four > 50, "four-wave kernels not found"
+    assert four > 40, "four-wave kernels not found"
+    assert queued == 6, "queued packed forwards: 2 output forms x 3 widths"
 
 
 _WIDE_STORE = re.compile(r"^(buffer|global|flat)_store_dwordx(3|4)\b.*?v\[(\d+):(\d+)\]")
