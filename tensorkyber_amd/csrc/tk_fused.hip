@@ -546,20 +546,31 @@ __global__ __launch_bounds__((64 * fused_waves<MODE, K>())) void k_fused(
 // The tiles of a workgroup go through the phases together (workgroup
 // barriers); tile t of iteration it belongs to sub-tile (t / grid) mod S of
 // workgroup t mod grid, so a batch of fewer tiles than CUs still spreads.
+//
+// A workgroup holds 12 waves (12 / K tiles: 6, 4, 3 for ranks 2, 3, 4),
+// three per SIMD.  Measured against the most tiles LDS allows (7 at rank 2,
+// 5 at rank 3: 14 and 15 waves) and the fused kernel at rank 4 (16 waves of
+// four-wave workgroups), per launch: rank 2 61.5 -> 55.6 us, rank 3 93.0 ->
+// 88.3 us, rank 4 141.1 -> 131.2 us at 65,536 instances, and 4-7% less at
+// 81,920 and 131,072 (rank 3: 109.4 -> 107.3 and 172.6 -> 165.6 us, where
+// five tiles per round come out even); 5 or 3 tiles at ranks 2 and 3 (10 or
+// 9 waves) measured slower again.
 // ---------------------------------------------------------------------------
 // biases of both per-block passes, fwd2 + inv2 matrices, fwd1 + inv1p
 // matrices and their biases
 constexpr int kWideLdsTables = 2 * 1024 + 2 * 16384 + 2 * 1024 + 2 * 64;
+constexpr int kWideWaves = 12;
 template <int K>
 constexpr int wide_tiles()
 {
-  constexpr int by_lds = (163840 - kWideLdsTables) / (K * kTileInst * kStride * 2);
-  return by_lds < 16 / K ? by_lds : 16 / K;
+  return kWideWaves / K;
 }
+static_assert(kWideLdsTables + (kWideWaves / 2) * 2 * kTileInst * kStride * 2 <= 163840,
+              "the wide kernels' slabs and tables fit the workgroup's LDS");
 constexpr int kWidePre = 3;  // matrix-row chunks in flight
 
 template <int K>
-__global__ __launch_bounds__(64 * K * wide_tiles<K>()) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ __launch_bounds__(64 * K * wide_tiles<K>()) __attribute__((amdgpu_waves_per_eu(3)))
 void k_matvec_wide(const FusedIo io, const DeviceTables *__restrict__ tab)
 {
   constexpr int P = kTileInst, S = wide_tiles<K>(), W = K * S;
@@ -1018,7 +1029,8 @@ int launch_fused(int k, int mode, int s_bits, const FusedIo &io, const DeviceTab
         switch (k) {
           case 2: return wide(k_matvec_wide<2>, 2 * wide_tiles<2>());
           case 3: return wide(k_matvec_wide<3>, 3 * wide_tiles<3>());
-          default: break;  // rank 4: three tiles per CU, fewer waves than k_fused keeps
+          case 4: return wide(k_matvec_wide<4>, 4 * wide_tiles<4>());
+          default: break;
         }
       switch (s_bits) {
         case 0: return by_table(mc);

@@ -5,8 +5,8 @@ compiler's resource remarks.  Two properties that GPU tests only catch by
 luck are checked on every kernel:
 
 * no spills to memory, and the tile kernels at their occupancy targets (four
-  waves per SIMD for the MFMA16 kernels and the wide fused kernels, two for
-  MFMA32): a spill or a lost wave costs 5-15% on the streaming kernels without
+  waves per SIMD for the MFMA16 kernels and the wide decryption, three for
+  the wide matvec, two for MFMA32): a spill or a lost wave costs 5-15% on the streaming kernels without
   changing any result;
 * no store of more than 8 bytes whose data VGPRs the very next VALU
   instruction overwrites.  On gfx950 such a store writes the new values; the
@@ -64,7 +64,7 @@ def _resources(remarks):
 
 
 def test_no_spills_and_tile_kernel_occupancy(compiled):
-    seen = four = queued = 0
+    seen = four = queued = wide_mv = 0
     for src, _, remarks in compiled:
         for name, r in _resources(remarks).items():
             seen += 1
@@ -73,7 +73,7 @@ def test_no_spills_and_tile_kernel_occupancy(compiled):
             assert r.get("ScratchSize [bytes/lane]", 0) == 0, f"{src}: {name} uses scratch"
             # (k_decrypt<K, TABLE, WIDE>, mangled ...Lb0ELb1E for the wide
             # variant: the table-driven one runs three)
-            wide = "k_matvec_wide" in name or ("k_decrypt" in name and "Lb0ELb1E" in name)
+            wide = "k_decrypt" in name and "Lb0ELb1E" in name
             if "k_mfma16" in name or wide:
                 assert r["Occupancy [waves/SIMD]"] >= 4, f"{name}: occupancy below 4"
                 four += 1
@@ -82,9 +82,13 @@ def test_no_spills_and_tile_kernel_occupancy(compiled):
                 queued += "ELi16ENS_11PackedTiles" in name
             if "k_mfma32" in name:
                 assert r["Occupancy [waves/SIMD]"] >= 2, f"{name}: occupancy below 2"
+            if "k_matvec_wide" in name:  # 12-wave workgroups, one per CU: three per SIMD
+                assert r["Occupancy [waves/SIMD]"] >= 3, f"{name}: occupancy below 3"
+                wide_mv += 1
     assert seen > 50, "kernel remarks not found"
     assert four > 40, "four-wave kernels not found"
     assert queued == 6, "queued packed forwards: 2 output forms x 3 widths"
+    assert wide_mv == 3, "wide matvec kernels for ranks 2, 3 and 4"
 
 
 _WIDE_STORE = re.compile(r"^(buffer|global|flat)_store_dwordx(3|4)\b.*?v\[(\d+):(\d+)\]")
