@@ -169,9 +169,9 @@ struct PackedBits<PackedTiles<BITS>> : std::integral_constant<int, BITS> {
 //
 // QUEUED: the waves of a workgroup take their tiles from a counter in LDS
 // instead of striding over them: workgroup b's k-th tile is k * grid + b,
-// handed to whichever of its waves asks next.  With one 16-wave workgroup
-// per CU (the launcher's shape for packed input) the four waves of a SIMD
-// then share its tiles.  Striding, each wave had the same number of tiles
+// handed to whichever of its waves asks next.  With one workgroup per CU
+// (the launcher's shape for packed input) the waves of a SIMD then share
+// its tiles.  Striding, each wave had the same number of tiles
 // while the SIMD issues for the oldest ready wave first: the four finished
 // in a staircase (2^20 2-bit polys: at about 60, 80, 100 and 118 us), and
 // the last third of the kernel ran on fewer waves than the SIMD holds.
@@ -341,7 +341,11 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2)))
   }
 }
 
-constexpr int kQueuedWaves = 16;  // one workgroup per CU, four waves per SIMD
+// Waves of a queued workgroup (one per CU): three per SIMD.  2^20 polys:
+// 116-118 us at 2 bits and 130 us at 3 bits, against 118-119 and 132-133
+// with 16 waves; 8 waves took 122 and 130 us, two 8-wave workgroups per CU
+// 129 and 136 us.
+constexpr int kQueuedWaves = 12;
 
 inline size_t host_tile_count(const LinearTiles &io, int P) { return (io.npoly + P - 1) / P; }
 inline size_t host_tile_count(const RaggedTiles &io, int) { return io.ntiles; }
@@ -378,8 +382,8 @@ int launch_one(int waves, int ncu, int wgs, const Tiles &io, const DeviceTables 
       return swap ? launch_persistent(swapped, nw, 16, ncu, wgs, io, tab, s)
                   : launch_persistent(plain, nw, 16, ncu, wgs, io, tab, s);
     };
-    // Packed input: one 16-wave workgroup per CU (its LDS), the waves
-    // taking tiles from the workgroup's queue (k_mfma16, QUEUED).
+    // Packed input: one workgroup per CU, the waves taking tiles from the
+    // workgroup's queue (k_mfma16, QUEUED).
     if constexpr (PackedBits<Tiles>::value != 0) {
       const size_t ntiles = host_tile_count(io, 16);
       const dim3 grid((unsigned)(ntiles < (size_t)ncu ? ntiles : (size_t)ncu));

@@ -6,7 +6,7 @@ luck are checked on every kernel:
 
 * no spills to memory, and the tile kernels at their occupancy targets (four
   waves per SIMD for the MFMA16 kernels and the wide decryption, three for
-  the wide matvec, two for MFMA32): a spill or a lost wave costs 5-15% on the streaming kernels without
+  the queued packed MFMA16 and the wide matvec, two for MFMA32): a spill or a lost wave costs 5-15% on the streaming kernels without
   changing any result;
 * no store of more than 8 bytes whose data VGPRs the very next VALU
   instruction overwrites.  On gfx950 such a store writes the new values; the
@@ -74,19 +74,21 @@ def test_no_spills_and_tile_kernel_occupancy(compiled):
             # (k_decrypt<K, TABLE, WIDE>, mangled ...Lb0ELb1E for the wide
             # variant: the table-driven one runs three)
             wide = "k_decrypt" in name and "Lb0ELb1E" in name
-            if "k_mfma16" in name or wide:
+            # the queued packed instances, one 12-wave workgroup per CU: three
+            # per SIMD (k_mfma16<FORM, INV, 12, PackedTiles<B>, true, true>)
+            if "k_mfma16" in name and "ELi12ENS_11PackedTiles" in name:
+                assert r["Occupancy [waves/SIMD]"] >= 3, f"{name}: occupancy below 3"
+                queued += 1
+            elif "k_mfma16" in name or wide:
                 assert r["Occupancy [waves/SIMD]"] >= 4, f"{name}: occupancy below 4"
                 four += 1
-                # the queued packed instances: 16 waves per workgroup, tiles from LDS
-                # (k_mfma16<FORM, INV, 16, PackedTiles<B>, true, true>)
-                queued += "ELi16ENS_11PackedTiles" in name
             if "k_mfma32" in name:
                 assert r["Occupancy [waves/SIMD]"] >= 2, f"{name}: occupancy below 2"
             if "k_matvec_wide" in name:  # 12-wave workgroups, one per CU: three per SIMD
                 assert r["Occupancy [waves/SIMD]"] >= 3, f"{name}: occupancy below 3"
                 wide_mv += 1
     assert seen > 50, "kernel remarks not found"
-    assert four > 40, "four-wave kernels not found"
+    assert four >= 30, "four-wave kernels not found"
     assert queued == 6, "queued packed forwards: 2 output forms x 3 widths"
     assert wide_mv == 3, "wide matvec kernels for ranks 2, 3 and 4"
 
