@@ -565,7 +565,8 @@ constexpr int wide_tiles()
 {
   return kWideWaves / K;
 }
-static_assert(kWideLdsTables + (kWideWaves / 2) * 2 * kTileInst * kStride * 2 <= 163840,
+// kWideWaves slabs (K divides 12 for every rank) and the tables.
+static_assert(kWideLdsTables + kWideWaves * kTileInst * kStride * 2 <= 163840,
               "the wide kernels' slabs and tables fit the workgroup's LDS");
 constexpr int kWidePre = 3;  // matrix-row chunks in flight
 
