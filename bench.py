@@ -297,7 +297,7 @@ def bench_ragged(tk, torch, steps):
     * ``rt_ms``: tk_ragged_plan_execute of the forward then the inverse plan
       (one dispatch each), `steps` round trips between two events;
     * ``fwd_ms`` / ``inv_ms``: each plan alone, per launch;
-    * ``rt_ms_graph_replay``: each plan's recorded graph (tk_ragged_plan_graph)
+    * ``rt_ms_graph_replay``: each plan's graph (tk_ragged_plan_graph)
       instantiated and launched, two graph launches per round trip;
     * ``rt_ms_one_graph``: both plans as child nodes of one graph, one launch
       per round trip (the cost of the graph launch paid once per batch)."""

@@ -114,7 +114,8 @@ typedef struct {
  * of the kernel's tile height (16 or 32 polys) and copies the table of tiles
  * ({first poly, number of polys}, 16 bytes per tile) to device memory once,
  * picks the kernel shape with the autotuner (or takes method MFMA16 / MFMA32
- * as given) and records the launch of the persistent kernel in a hipGraph.
+ * as given) and builds a one-node hipGraph of the persistent kernel's launch
+ * (a kernel node, not a stream capture).
  * direction: 0 = forward (canonical out), 1 = inverse (centred out).
  * Transforms are in place.  Plan creation allocates and synchronises; the
  * execute and launch calls do neither.  Requests must not overlap one
@@ -129,7 +130,7 @@ int tk_ragged_plan_create(tk_ragged_plan **plan, const tk_request *table, size_t
 int tk_ragged_plan_execute(tk_ragged_plan *plan, void *stream);
 /* The same as tk_ragged_plan_execute. */
 int tk_ragged_plan_launch(tk_ragged_plan *plan, void *stream);
-/* The plan's recorded launch as a hipGraph_t (owned by the plan, valid until
+/* The plan's launch as a hipGraph_t (owned by the plan, valid until
  * it is destroyed), for callers that build a graph of their own work and add
  * plans to it as child graph nodes (hipGraphAddChildGraphNode copies it). */
 int tk_ragged_plan_graph(tk_ragged_plan *plan, void **graph);

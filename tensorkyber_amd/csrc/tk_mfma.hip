@@ -226,7 +226,7 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(4)))
   // staging registers as soon as the first pass has consumed them.  Both
   // measured faster than loading after the stores: 180-182 against 183 us
   // per 2^20-poly int16 pass, 126-130 against 138-140 us for the 2-bit
-  // packed forward.  vm_drain keeps the loop's waits from covering the
+  // packed forward (before its tile queue).  vm_drain keeps the loop's waits from covering the
   // prologue's loads.  Without SWAP (one tile per wave) the int16 tile is
   // staged in at the top of the loop and the prologue is not drained, so the
   // matrices' loads stay in flight through the tile's first pass.
