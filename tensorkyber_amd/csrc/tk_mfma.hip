@@ -146,11 +146,11 @@ __device__ __forceinline__ void stage_in(const PackedTiles<BITS> &, const TileIo
     packed_park<P, BITS>(L, r.v, lane);
 }
 
-// Write the slab to the tile's output polys.
-template <int P>
+// Write the slab to the tile's output polys, with cache policy POL.
+template <int P, int POL>
 __device__ __forceinline__ void stage_out(const TileIo &t, const int16_t *L, int lane)
 {
-  tile_out_rsrc<P>(L, span_rsrc(t.out, t.npoly * 2 * kN), lane);
+  tile_out_rsrc<P, POL>(L, span_rsrc(t.out, t.npoly * 2 * kN), lane);
 }
 
 // Bits per coefficient of packed tiles, 0 for int16 tiles.
@@ -175,7 +175,11 @@ struct PackedBits<PackedTiles<BITS>> : std::integral_constant<int, BITS> {
 // while the SIMD issues for the oldest ready wave first: the four finished
 // in a staircase (2^20 2-bit polys: at about 60, 80, 100 and 118 us), and
 // the last third of the kernel ran on fewer waves than the SIMD holds.
-template <int FORM, bool INV, int WAVES, class Tiles, bool SWAP, bool QUEUED = false>
+//
+// KEEP: the output is stored with the default cache policy instead of
+// nontemporally, so that it stays in the Infinity Cache (keep_output).
+template <int FORM, bool INV, int WAVES, class Tiles, bool SWAP, bool QUEUED = false,
+          bool KEEP = false>
 __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(4))) void k_mfma16(Tiles io,
                                                         const DeviceTables *__restrict__ tab)
 {
@@ -242,6 +246,7 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(4)))
   TileIo cur = tile_io<P>(io, tile < ntiles ? tile : 0);
   StageRegs<P, Tiles> sr;
   constexpr bool kSwap = PB == 0 && SWAP;
+  constexpr int kOut = KEEP ? kBufferKeep : kBufferNT;
   stage_load<P>(io, tile, sr, lane);
   if constexpr (kSwap)
     if (tile < ntiles)
@@ -273,17 +278,17 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(4)))
     }
     wave_sync();
     if (kSwap && more)
-      tile_swap_rsrc16(L, span_rsrc(cur.out, cur.npoly * 2 * kN),
-                       span_rsrc(nxt.in, nxt.npoly * 2 * kN), lane);
+      tile_swap_rsrc16<kOut>(L, span_rsrc(cur.out, cur.npoly * 2 * kN),
+                             span_rsrc(nxt.in, nxt.npoly * 2 * kN), lane);
     else
-      stage_out<P>(cur, L, lane);
+      stage_out<P, kOut>(cur, L, lane);
     wave_sync();
     cur = nxt;
   }
 }
 
 // Two waves per SIMD (at most 256 VGPRs), as for k_mfma16.
-template <int FORM, bool INV, int WAVES, class Tiles>
+template <int FORM, bool INV, int WAVES, class Tiles, bool KEEP = false>
 __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2))) void k_mfma32(Tiles io,
                                                         const DeviceTables *__restrict__ tab)
 {
@@ -335,7 +340,7 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2)))
     }
     wave_sync();
     stage_load<P>(io, tile + stride, sr, lane);
-    stage_out<P>(cur, L, lane);
+    stage_out<P, KEEP ? kBufferKeep : kBufferNT>(cur, L, lane);
     wave_sync();
     cur = nxt;
   }
@@ -368,9 +373,42 @@ int launch_persistent(K kernel, int waves, int P, int ncu, int oversub, const Ti
   return sink_launch(s, kernel, grid, dim3(block), io, tab);
 }
 
-template <int SHAPE, int FORM, bool INV, class Tiles>
-int launch_one(int waves, int ncu, int wgs, const Tiles &io, const DeviceTables *tab,
-               const LaunchSink &s)
+// Output cache policy of linear int16 batches.  Nontemporal stores stream
+// the output past the caches; stores with the default policy leave it in the
+// MI355X's 256 MB Infinity Cache, where the batch's next pass (the inverse
+// after the forward, or the next transform in place) reads it instead of
+// HBM.  That pays while the batch fits.  Measured, forward + inverse in
+// place, MFMA16, per round trip:
+//
+//   polys        nontemporal   kept
+//   32,768          17.7 us    20.1 us
+//   65,536          34.5       29.8
+//   262,144        106.3       92.6
+//   524,288        201.9      166.1
+//   589,824        219.9      241.0
+//   2^20           361.1      382.2
+//
+// and, each pass timed alone after staging the batch with a device copy,
+// 106.9 against 98.2 us at 262,144 polys.  MFMA32 gains from 98,304 polys
+// in place (45.3 against 42.6 us) but loses 1-2.5 us per round trip below
+// 131,072 polys when each pass follows a copy (65,536: 35.7 against 38.2).
+// Batches of 32 MiB (MFMA16) or 64 MiB (MFMA32) up to 256 MiB keep their
+// output.
+constexpr size_t kKeepMaxPolys = 524288;
+
+inline bool keep_output(const LinearTiles &io, int shape)
+{
+  return io.npoly >= (shape == 16 ? 65536u : 131072u) && io.npoly <= kKeepMaxPolys;
+}
+template <class Tiles>
+inline bool keep_output(const Tiles &, int)
+{
+  return false;
+}
+
+template <int SHAPE, int FORM, bool INV, bool KEEP, class Tiles>
+int launch_policy(int waves, int ncu, int wgs, const Tiles &io, const DeviceTables *tab,
+                  const LaunchSink &s)
 {
   if (SHAPE == 16) {
     // The swap pays when a wave has more than one tile: the grid (capped at
@@ -390,13 +428,25 @@ int launch_one(int waves, int ncu, int wgs, const Tiles &io, const DeviceTables 
       return sink_launch(s, k_mfma16<FORM, INV, kQueuedWaves, Tiles, true, true>, grid,
                          dim3(64 * kQueuedWaves), io, tab);
     } else {
-      return waves == 2
-                 ? pick(k_mfma16<FORM, INV, 2, Tiles, true>, k_mfma16<FORM, INV, 2, Tiles, false>, 2)
-                 : pick(k_mfma16<FORM, INV, 4, Tiles, true>, k_mfma16<FORM, INV, 4, Tiles, false>, 4);
+      return waves == 2 ? pick(k_mfma16<FORM, INV, 2, Tiles, true, false, KEEP>,
+                               k_mfma16<FORM, INV, 2, Tiles, false, false, KEEP>, 2)
+                        : pick(k_mfma16<FORM, INV, 4, Tiles, true, false, KEEP>,
+                               k_mfma16<FORM, INV, 4, Tiles, false, false, KEEP>, 4);
     }
   }
-  return waves == 2 ? launch_persistent(k_mfma32<FORM, INV, 2, Tiles>, 2, 32, ncu, wgs, io, tab, s)
-                    : launch_persistent(k_mfma32<FORM, INV, 4, Tiles>, 4, 32, ncu, wgs, io, tab, s);
+  return waves == 2
+             ? launch_persistent(k_mfma32<FORM, INV, 2, Tiles, KEEP>, 2, 32, ncu, wgs, io, tab, s)
+             : launch_persistent(k_mfma32<FORM, INV, 4, Tiles, KEEP>, 4, 32, ncu, wgs, io, tab, s);
+}
+
+template <int SHAPE, int FORM, bool INV, class Tiles>
+int launch_one(int waves, int ncu, int wgs, const Tiles &io, const DeviceTables *tab,
+               const LaunchSink &s)
+{
+  if constexpr (std::is_same<Tiles, LinearTiles>::value)
+    if (keep_output(io, SHAPE))
+      return launch_policy<SHAPE, FORM, INV, true>(waves, ncu, wgs, io, tab, s);
+  return launch_policy<SHAPE, FORM, INV, false>(waves, ncu, wgs, io, tab, s);
 }
 
 template <int SHAPE, bool INV, class Tiles>

@@ -37,6 +37,7 @@ constexpr int kPassGroup = 4;
 // allocation read nothing past it (tests/test_gpu_bounds.py).
 constexpr int kBufferRsrcWord3 = 0x00020000;  // gfx9 buffer descriptor: 32-bit data format
 constexpr int kBufferNT = 2;                  // cache policy: nontemporal
+constexpr int kBufferKeep = 0;                // cache policy: default (allocates in the caches)
 
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t span_rsrc(const void *base, uint32_t bytes)
 {
@@ -67,7 +68,9 @@ __device__ __forceinline__ void tile_in_rsrc(int16_t *L, __amdgpu_buffer_rsrc_t 
   }
 }
 
-template <int P>
+// POL: the stores' cache policy (kBufferNT, or kBufferKeep to leave the
+// tile in the caches for the batch's next pass).
+template <int P, int POL = kBufferNT>
 __device__ __forceinline__ void tile_out_rsrc(const int16_t *L, __amdgpu_buffer_rsrc_t r, int lane)
 {
   typedef unsigned int v4u __attribute__((ext_vector_type(4)));
@@ -82,7 +85,7 @@ __device__ __forceinline__ void tile_out_rsrc(const int16_t *L, __amdgpu_buffer_
 #pragma unroll
     for (int i = 0; i < 8; i++)
       __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4u, v[i]), r,
-                                             lane * 16 + (g0 + i) * 1024, 0, kBufferNT);
+                                             lane * 16 + (g0 + i) * 1024, 0, POL);
     if (g0 + 8 < P / 2)
       __builtin_amdgcn_sched_barrier(0);
   }
@@ -94,7 +97,9 @@ __device__ __forceinline__ void tile_out_rsrc(const int16_t *L, __amdgpu_buffer_
 // (vector memory operations complete in issue order); the stores complete
 // during the next tile's passes.  The stores' data goes through registers
 // two chunks at a time (a scheduling barrier per pair), so the swap holds 8
-// registers beyond the loads' 32 and the kernel keeps 128 VGPRs.
+// registers beyond the loads' 32 and the kernel keeps 128 VGPRs.  POL is the
+// stores' cache policy, as for tile_out_rsrc.
+template <int POL = kBufferNT>
 __device__ __forceinline__ void tile_swap_rsrc16(int16_t *L, __amdgpu_buffer_rsrc_t rout,
                                                  __amdgpu_buffer_rsrc_t rin, int lane)
 {
@@ -115,7 +120,7 @@ __device__ __forceinline__ void tile_swap_rsrc16(int16_t *L, __amdgpu_buffer_rsr
 #pragma unroll
     for (int i = 0; i < 2; i++)
       __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4u, o[i]), rout,
-                                             lane * 16 + (h + i) * 1024, 0, kBufferNT);
+                                             lane * 16 + (h + i) * 1024, 0, POL);
     __builtin_amdgcn_sched_barrier(0);
   }
 #pragma unroll

@@ -176,3 +176,27 @@ def test_mfma16_tile_swap_sizes(tk, gpu, oracle, npoly):
     assert np.array_equal(_host(inv), centred(oracle.invntt(want_f))), "inverse in place"
     assert np.array_equal(_host(src), x), "input changed"
     torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("method,npoly", [("mfma16", 65535), ("mfma16", 65536),
+                                          ("mfma16", 524288), ("mfma32", 131071),
+                                          ("mfma32", 131072), ("mfma32", 524288)])
+def test_kept_output_policy_edges(tk, gpu, oracle, method, npoly):
+    """Linear batches of 65,536 (MFMA16) or 131,072 (MFMA32) to 524,288 polys
+    store their output with the default cache policy (it stays in the
+    Infinity Cache for the next pass), others nontemporally: both sides of the
+    lower edge and the upper edge (the next size up is covered above), out of
+    place and in place, against the oracle on every poly."""
+    import torch
+    print(f"output policy edge {method} npoly={npoly}", flush=True)
+    x = _inputs(tk, "edge", npoly, 80 + npoly % 89)
+    src = _dev(x, gpu)
+    fwd = tk.ntt(src, out=torch.empty_like(src), method=method, form="canonical")
+    want_f = canon(oracle.ntt(x))
+    assert np.array_equal(_host(fwd), want_f), "forward out of place"
+    assert np.array_equal(_host(src), x), "input changed"
+    tk.ntt(src, out=src, method=method, form="canonical")
+    assert np.array_equal(_host(src), want_f), "forward in place"
+    inv = tk.invntt(fwd, out=torch.empty_like(fwd), method=method, form="centered")
+    assert np.array_equal(_host(inv), centred(oracle.invntt(want_f))), "inverse"
+    torch.cuda.synchronize()
