@@ -942,10 +942,21 @@ __attribute__((amdgpu_waves_per_eu(TABLE ? 3 : 4))) void k_decrypt(
         __builtin_amdgcn_sched_barrier(0);
       }
     };
+    // Wide: the products run at wave priority 1 and the inverse and the
+    // epilogue at 0, so a SIMD issues first for the waves still streaming
+    // u_j and s_hat_j in, and the waves near their tile's end fill the gaps.
+    // 65,536 instances, one process: 49.6-50.0 against 53.7-54.0 us at rank
+    // 3, 39.5 against 42.1 at rank 2, 60.7 against 64.5 at rank 4.  (Every
+    // phase at its own level, 3 down to 0, took rank 3 to 66 us: waves kept
+    // in step load and compute at the same moments.)
+    if constexpr (WIDE)
+      __builtin_amdgcn_s_setprio(1);
 #pragma unroll 1
     for (int j = 0; j < K - 1; j++)
       step(j, std::false_type{});
     step(K - 1, std::true_type{});
+    if constexpr (WIDE)
+      __builtin_amdgcn_s_setprio(0);
 
     // The row into the slab (its reads above are done), inverse-transformed.
     wave_sync();
