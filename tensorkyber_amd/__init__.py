@@ -169,10 +169,21 @@ def _torch():
     return torch
 
 
-def _stream_ptr(stream=None):
+def _stream_ptr(stream=None, device=None):
+    """Raw handle of `stream`, or of the current stream of `device` (the
+    tensors' device, which must be the current device: the C ABI launches on
+    the current device).  torch.cuda.current_stream() builds a Stream object
+    per call (about 2.7 us of host time, which a synchronised single call
+    exposes), so the current stream's handle is read directly where torch
+    offers that."""
     torch = _torch()
-    s = stream if stream is not None else torch.cuda.current_stream()
-    return ctypes.c_void_p(s.cuda_stream)
+    cur = torch.cuda.current_device()
+    if device is not None and device.index != cur:
+        raise ValueError(f"tensors on {device}, but the current device is cuda:{cur}")
+    if stream is not None:
+        return stream.cuda_stream
+    raw = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+    return raw(cur) if raw is not None else torch.cuda.current_stream(cur).cuda_stream
 
 
 def _as_polys(t, name="polys"):
@@ -185,12 +196,19 @@ def _as_polys(t, name="polys"):
 
 
 def _one_device(**tensors):
-    """All given tensors (None skipped) on one device: the C ABI takes raw
-    pointers and launches on the current device."""
-    devs = {name: t.device for name, t in tensors.items() if t is not None}
-    if len(set(devs.values())) > 1:
-        raise ValueError("tensors on different devices: " +
-                         ", ".join(f"{n} on {d}" for n, d in devs.items()))
+    """The one device of the given tensors (None skipped): the C ABI takes
+    raw pointers and launches on one device."""
+    dev = None
+    for t in tensors.values():
+        if t is None:
+            continue
+        if dev is None:
+            dev = t.device
+        elif t.device != dev:
+            raise ValueError("tensors on different devices: " +
+                             ", ".join(f"{n} on {t.device}" for n, t in tensors.items()
+                                       if t is not None))
+    return dev
 
 
 def _same_size(out, ref, name="out", ref_name="the input"):
@@ -213,9 +231,9 @@ def ntt(polys, out=None, method="auto", form="default", stream=None):
     polys = _as_polys(polys)
     out = polys if out is None else _as_polys(out, "out")
     _same_size(out, polys)
-    _one_device(polys=polys, out=out)
+    dev = _one_device(polys=polys, out=out)
     _check(lib().ntt_batch_ex(out.data_ptr(), polys.data_ptr(), polys.numel() // N,
-                              _method(method), _form(form), _stream_ptr(stream)), "ntt_batch")
+                              _method(method), _form(form), _stream_ptr(stream, dev)), "ntt_batch")
     return out
 
 
@@ -224,9 +242,10 @@ def invntt(polys, out=None, method="auto", form="default", stream=None):
     polys = _as_polys(polys)
     out = polys if out is None else _as_polys(out, "out")
     _same_size(out, polys)
-    _one_device(polys=polys, out=out)
+    dev = _one_device(polys=polys, out=out)
     _check(lib().invntt_batch_ex(out.data_ptr(), polys.data_ptr(), polys.numel() // N,
-                                 _method(method), _form(form), _stream_ptr(stream)), "invntt_batch")
+                                 _method(method), _form(form), _stream_ptr(stream, dev)),
+           "invntt_batch")
     return out
 
 
@@ -238,9 +257,9 @@ def basemul(a, b, out=None, form="default", stream=None):
     torch = _torch()
     out = torch.empty_like(a) if out is None else _as_polys(out, "out")
     _same_size(out, a, ref_name="a")
-    _one_device(a=a, b=b, out=out)
+    dev = _one_device(a=a, b=b, out=out)
     _check(lib().basemul_batch_ex(out.data_ptr(), a.data_ptr(), b.data_ptr(), a.numel() // N,
-                                  _form(form), _stream_ptr(stream)), "basemul_batch")
+                                  _form(form), _stream_ptr(stream, dev)), "basemul_batch")
     return out
 
 
@@ -254,9 +273,9 @@ def matvec(A_hat, s, k: int, out=None, stream=None):
     out = torch.empty_like(s) if out is None else _as_polys(out, "out")
     if s.numel() != ninst * k * N or out.numel() != s.numel():
         raise ValueError("s and out must hold ninst*k polys")
-    _one_device(A_hat=A_hat, s=s, out=out)
+    dev = _one_device(A_hat=A_hat, s=s, out=out)
     _check(lib().matvec_ntt_fused(out.data_ptr(), A_hat.data_ptr(), s.data_ptr(), k, ninst,
-                                  _stream_ptr(stream)), "matvec_ntt_fused")
+                                  _stream_ptr(stream, dev)), "matvec_ntt_fused")
     return out
 
 
@@ -289,9 +308,9 @@ def pack_small(polys, bits: int, out=None, stream=None):
     out, ppb = _as_packed(out, bits, "out")
     if out.numel() != npoly * ppb:
         raise ValueError("out has the wrong size")
-    _one_device(polys=polys, out=out)
+    dev = _one_device(polys=polys, out=out)
     _check(lib().pack_small_batch(out.data_ptr(), polys.data_ptr(), npoly, bits,
-                                  _stream_ptr(stream)), "pack_small_batch")
+                                  _stream_ptr(stream, dev)), "pack_small_batch")
     return out
 
 
@@ -304,9 +323,9 @@ def ntt_packed(packed, bits: int, out=None, method="auto", form="default", strea
         else _as_polys(out, "out")
     if out.numel() != npoly * N:
         raise ValueError("out has the wrong size")
-    _one_device(packed=packed, out=out)
+    dev = _one_device(packed=packed, out=out)
     _check(lib().ntt_batch_packed(out.data_ptr(), packed.data_ptr(), npoly, bits, _method(method),
-                                  _form(form), _stream_ptr(stream)), "ntt_batch_packed")
+                                  _form(form), _stream_ptr(stream, dev)), "ntt_batch_packed")
     return out
 
 
@@ -322,9 +341,9 @@ def matvec_packed(A_hat, s_packed, k: int, bits: int, out=None, stream=None):
         else _as_polys(out, "out")
     if out.numel() != ninst * k * N:
         raise ValueError("out must hold ninst*k polys")
-    _one_device(A_hat=A_hat, s_packed=s_packed, out=out)
+    dev = _one_device(A_hat=A_hat, s_packed=s_packed, out=out)
     _check(lib().matvec_ntt_fused_packed(out.data_ptr(), A_hat.data_ptr(), s_packed.data_ptr(), k,
-                                         ninst, bits, _stream_ptr(stream)),
+                                         ninst, bits, _stream_ptr(stream, dev)),
            "matvec_ntt_fused_packed")
     return out
 
@@ -390,9 +409,9 @@ def kpke_keygen(A_hat, s, e, k: int, t_hat=None, s_hat=None, stream=None):
     s_hat = torch.empty_like(s) if s_hat is None else _as_polys(s_hat, "s_hat")
     if t_hat.numel() != s.numel() or s_hat.numel() != s.numel():
         raise ValueError("t_hat and s_hat must hold ninst*k polys")
-    _one_device(A_hat=A_hat, s=s, e=e, t_hat=t_hat, s_hat=s_hat)
+    dev = _one_device(A_hat=A_hat, s=s, e=e, t_hat=t_hat, s_hat=s_hat)
     _check(lib().kpke_keygen_core(t_hat.data_ptr(), s_hat.data_ptr(), A_hat.data_ptr(),
-                                  s.data_ptr(), e.data_ptr(), k, ninst, _stream_ptr(stream)),
+                                  s.data_ptr(), e.data_ptr(), k, ninst, _stream_ptr(stream, dev)),
            "kpke_keygen_core")
     return t_hat, s_hat
 
@@ -413,10 +432,10 @@ def kpke_encrypt(At_hat, t_hat, r, e1, e2, m, k: int, u=None, v=None, stream=Non
         else _as_polys(v, "v")
     if u.numel() != ninst * k * N or v.numel() != ninst * N:
         raise ValueError("u must hold ninst*k polys and v ninst polys")
-    _one_device(At_hat=At_hat, t_hat=t_hat, r=r, e1=e1, e2=e2, m=m, u=u, v=v)
+    dev = _one_device(At_hat=At_hat, t_hat=t_hat, r=r, e1=e1, e2=e2, m=m, u=u, v=v)
     _check(lib().kpke_encrypt_core(u.data_ptr(), v.data_ptr(), At_hat.data_ptr(), t_hat.data_ptr(),
                                    r.data_ptr(), e1.data_ptr(), e2.data_ptr(), _ptr_or_none(m), k,
-                                   ninst, _stream_ptr(stream)), "kpke_encrypt_core")
+                                   ninst, _stream_ptr(stream, dev)), "kpke_encrypt_core")
     return u, v
 
 
@@ -431,9 +450,9 @@ def kpke_decrypt(s_hat, u, v, k: int, w=None, stream=None):
         else _as_polys(w, "w")
     if w.numel() != ninst * N:
         raise ValueError("w must hold ninst polys")
-    _one_device(s_hat=s_hat, u=u, v=v, w=w)
+    dev = _one_device(s_hat=s_hat, u=u, v=v, w=w)
     _check(lib().kpke_decrypt_core(w.data_ptr(), s_hat.data_ptr(), u.data_ptr(), v.data_ptr(), k,
-                                   ninst, _stream_ptr(stream)), "kpke_decrypt_core")
+                                   ninst, _stream_ptr(stream, dev)), "kpke_decrypt_core")
     return w
 
 

@@ -31,7 +31,7 @@ def tkw(monkeypatch):
 
     monkeypatch.setattr(tk, "_as_polys", as_polys)
     monkeypatch.setattr(tk, "lib", lambda: _NoCalls())
-    monkeypatch.setattr(tk, "_stream_ptr", lambda stream=None: None)
+    monkeypatch.setattr(tk, "_stream_ptr", lambda stream=None, device=None: None)
     return tk
 
 
