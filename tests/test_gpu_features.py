@@ -37,16 +37,20 @@ def _stream():
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
+@pytest.mark.parametrize("total", [20000, 150000])
 @pytest.mark.parametrize("method", ["auto", "mfma16", "mfma32"])
-def test_ragged_plan_small(tk, gpu, oracle, method):
+def test_ragged_plan_small(tk, gpu, oracle, method, total):
+    """Forward and inverse plans against the oracle, gaps untouched; at
+    150,000 polys the plans store their output with the default cache policy
+    (kept in the Infinity Cache), at 20,000 nontemporally."""
     import torch
-    reqs, pool = tk.wl_ragged_layout(20000, 21)
+    reqs, pool = tk.wl_ragged_layout(total, 21)
     x = tk.wl_uniform(pool * N, 21).reshape(pool, N)
     d = _dev(x, gpu)
     plan = _make_plan(tk, d, reqs, 0, tk.METHODS[method])
     info = [ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_size_t()]
     assert tk.lib().tk_ragged_plan_info(plan, *[ctypes.byref(v) for v in info]) == 0
-    assert info[3].value == 20000 and info[1].value in (16, 32)
+    assert info[3].value == total and info[1].value in (16, 32)
     print(f"ragged plan {method}: shape {info[1].value} oversub {info[2].value}", flush=True)
     assert tk.lib().tk_ragged_plan_execute(plan, _stream()) == 0
     torch.cuda.synchronize()
