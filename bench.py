@@ -742,11 +742,12 @@ def load_traffic(kernel, npoly):
 # Kernel name -> bench kernel label (forward canonical, inverse centred, as
 # ntt_batch / invntt_batch launch them).
 # The 2^20-poly batch gives each MFMA16 wave two tiles: the tile-swap
-# instance (fifth template argument true) runs.
-KERNEL_LABELS = (("k_mfma16<1, false, 4, tk::LinearTiles, true", "mfma16_fwd"),
-                 ("k_mfma16<2, true, 4, tk::LinearTiles, true", "mfma16_inv"),
-                 ("k_mfma32<1, false, 4, tk::LinearTiles>", "mfma32_fwd"),
-                 ("k_mfma32<2, true, 4, tk::LinearTiles>", "mfma32_inv"),
+# instance (fifth template argument true) runs, with nontemporal stores (the
+# last argument false: the batch is larger than the Infinity Cache).
+KERNEL_LABELS = (("k_mfma16<1, false, 4, tk::LinearTiles, true, false, false>", "mfma16_fwd"),
+                 ("k_mfma16<2, true, 4, tk::LinearTiles, true, false, false>", "mfma16_inv"),
+                 ("k_mfma32<1, false, 4, tk::LinearTiles, false>", "mfma32_fwd"),
+                 ("k_mfma32<2, true, 4, tk::LinearTiles, false>", "mfma32_inv"),
                  ("k_ntt_butterfly", "butterfly_fwd"), ("k_invntt_butterfly", "butterfly_inv"))
 
 
