@@ -95,12 +95,12 @@ struct PackedTiles {
 // 32-poly tiles (180-183 / 184-188 us forward / inverse, against 188-190 /
 // 190-192 at 8; the inverse slows again from 16).  Packed batches load few
 // bytes per tile: 32-poly tiles use 1, with workgroups of 2 waves for 2-bit
-// input and 4 for 3 and 4 bits; 16-poly tiles run one 12-wave workgroup per
+// input and 4 for 3 and 4 bits; 16-poly tiles run one 16-wave workgroup per
 // CU whose waves take tiles from a queue in LDS (k_mfma16, QUEUED): 2^20
-// polys take 116-118 us at 2 bits and 130 us at 3 or 4, where the 1x grid
-// of 2- or 4-wave workgroups striding over the tiles took 126 and 135 us
-// (and 4x 141-143 us at 2 bits).  Ragged plans let the autotuner
-// choose, starting from the linear shape.
+// polys take 104-105 us at 2 bits and 108 us at 3, where the 1x grid of 2-
+// or 4-wave workgroups striding over the tiles took 126 and 135 us (and 4x
+// 141-143 us at 2 bits) with nontemporal loads.  Ragged plans let the
+// autotuner choose, starting from the linear shape.
 constexpr int kLinearOversub = 8;
 constexpr int kLinearOversub32 = 12;
 constexpr int kPackedOversub = 1;

@@ -346,11 +346,12 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2)))
   }
 }
 
-// Waves of a queued workgroup (one per CU): three per SIMD.  2^20 polys:
-// 116-118 us at 2 bits and 130 us at 3 bits, against 118-119 and 132-133
-// with 16 waves; 8 waves took 122 and 130 us, two 8-wave workgroups per CU
-// 129 and 136 us.
-constexpr int kQueuedWaves = 12;
+// Waves of a queued workgroup (one per CU): four per SIMD.  With the packed
+// loads kept in L2 (packed_load_lanes), 2^20 polys take 104-105 us at 2 bits
+// and 108 us at 3 bits, against 107-108 and 110 us with 12 waves and 120-124
+// with 8.  (With nontemporal loads 12 waves had been best: 116-118 and 130
+// us, against 118-119 and 132-133 with 16.)
+constexpr int kQueuedWaves = 16;
 
 inline size_t host_tile_count(const LinearTiles &io, int P) { return (io.npoly + P - 1) / P; }
 inline size_t host_tile_count(const RaggedTiles &io, int) { return io.ntiles; }
