@@ -269,16 +269,15 @@ __device__ __forceinline__ void role_out(const int16_t *slab, const RT &rt, int 
 }
 
 // K: module rank; MODE: kFusedMatvec / Keygen / Encrypt (FusedIo
-// documents the buffers of each); SBITS = 0: the transformed vector holds
-// int16 polys [ninst][K][256]; 2, 3, 4: the packed small-coefficient form
-// [ninst][K][packed bytes].  TABLE: each instance's buffers come from the
-// per-instance table io.inst (int16 x only); the tile's 16 rows of pointers
-// are parked in LDS, double-buffered by tile parity.
-template <int K, int MODE, int SBITS, bool TABLE>
+// documents the buffers of each); the transformed vector holds int16 polys
+// [ninst][K][256].  Contiguous matvec batches, int16 or packed, run
+// k_matvec_wide instead; this kernel takes them from plans.  TABLE: each
+// instance's buffers come from the per-instance table io.inst; the tile's
+// 16 rows of pointers are parked in LDS, double-buffered by tile parity.
+template <int K, int MODE, bool TABLE>
 __global__ __launch_bounds__((64 * fused_waves<MODE, K>())) void k_fused(
     const FusedIo io, const DeviceTables *__restrict__ tab)
 {
-  static_assert(!TABLE || SBITS == 0, "table-driven launches take int16 vectors");
   constexpr int P = kTileInst, W = fused_waves<MODE, K>(), R = fused_rows<MODE, K>();
   __shared__ const int16_t *s_ip[TABLE ? 2 : 1][TABLE ? P : 1][8];
   __shared__ __align__(16) int16_t s_slab[W][P * kStride];
@@ -296,12 +295,6 @@ __global__ __launch_bounds__((64 * fused_waves<MODE, K>())) void k_fused(
   const long f1lo = ld_a(tab->m16_fwd1[0], 0, lane), f1hi = ld_a(tab->m16_fwd1[0], 1, lane);
   const long i1lo = ld_a(tab->m16_inv1p[0], 0, lane), i1hi = ld_a(tab->m16_inv1p[0], 1, lane);
   const v4i fb1 = *reinterpret_cast<const v4i *>(&tab->bias_fwd1[0][4 * g]);
-  // Packed secrets run the first pass straight from their (field, sign bit)
-  // digits (pass_m16_packed).
-  constexpr int kPk = SBITS ? SBITS - 2 : 0;
-  const long f1pklo = SBITS ? ld_a(tab->m16_fwd1pk[kPk], 0, lane) : 0;
-  const long f1pkhi = SBITS ? ld_a(tab->m16_fwd1pk[kPk], 1, lane) : 0;
-  const v4i fb1pk = *reinterpret_cast<const v4i *>(&tab->bias_fwd1pk[0][4 * g]);
   const v4i ib1 = *reinterpret_cast<const v4i *>(&tab->bias_inv1p[0][4 * g]);
   // The per-block matrices of both directions are fetched at each use from
   // the (cache-resident) table through a buffer descriptor: one lane-offset
@@ -375,34 +368,8 @@ __global__ __launch_bounds__((64 * fused_waves<MODE, K>())) void k_fused(
 
     // 1. x_hat[w] = ntt(x_w), canonical, and its gamma products.
     if (w < K) {
-      if constexpr (SBITS == 0) {
-        role_in<P>(slab, role(rX), w, lane);
-        forward();
-      } else {
-        // Lane (g, t) takes the packed bytes of rows 4g..4g+3 of poly w of
-        // instance t (packed_load_lanes' layout), through a descriptor over
-        // the tile's valid instances, with the default cache policy as
-        // packed_load_lanes (65,536 rank-3 instances: 84.6-86.1 against
-        // 86.4-87.5 us nontemporal at 3 bits).
-        constexpr int PPB = packed_poly_bytes<SBITS>(), NDW = packed_lane_dwords<SBITS>();
-        const __amdgpu_buffer_rsrc_t xr =
-            span_rsrc(static_cast<const uint8_t *>(io.x) + i0 * K * PPB, (uint32_t)nvalid * K * PPB);
-        uint32_t pv[NDW];
-#pragma unroll
-        for (int i = 0; i < NDW / 4; i++) {
-          const v4i x = __builtin_bit_cast(
-              v4i, __builtin_amdgcn_raw_buffer_load_b128(xr, tt * K * PPB + g * NDW * 4,
-                                                         w * PPB + i * 16, kBufferKeep));
-          pv[4 * i] = (uint32_t)x[0];
-          pv[4 * i + 1] = (uint32_t)x[1];
-          pv[4 * i + 2] = (uint32_t)x[2];
-          pv[4 * i + 3] = (uint32_t)x[3];
-        }
-        pass_m16_packed<SBITS>(pslab, g, pv, f1pklo, f1pkhi, fb1pk);
-        wave_sync();
-        pass_c16<kFormInter>(pslab, g, f2lo, f2hi, s_bf2);
-        wave_sync();
-      }
+      role_in<P>(slab, role(rX), w, lane);
+      forward();
     }
     __syncthreads();
 
@@ -534,7 +501,7 @@ __global__ __launch_bounds__((64 * fused_waves<MODE, K>())) void k_fused(
 }
 
 // ---------------------------------------------------------------------------
-// Wide matvec: the fused product (contiguous buffers, int16 s) with one
+// Wide matvec: the fused product (contiguous buffers, s int16 or packed) with one
 // workgroup of several K-wave tiles per CU, so that the transform matrices
 // of both directions fit once in LDS for all of them.  The transforms then
 // issue no vector memory load, and since vector memory loads complete in
@@ -572,7 +539,15 @@ static_assert(kWideLdsTables + kWideWaves * kTileInst * kStride * 2 <= 163840,
               "the wide kernels' slabs and tables fit the workgroup's LDS");
 constexpr int kWidePre = 3;  // matrix-row chunks in flight
 
-template <int K>
+// SBITS = 2, 3, 4: s packed (matvec_ntt_fused_packed).  Each wave holds the
+// packed bytes of its s_w in the lane layout of the direct first pass
+// (pass_m16_packed), loaded with the default cache policy as
+// packed_load_lanes does, instead of the int16 rows; the first-pass matrix
+// and bias in LDS are the packed digits' (m16_fwd1pk) in place of fwd1's.
+// 65,536 instances, against the fused kernel's four-wave workgroups: rank 3
+// 84.4 against 88.8 us (3 bits), 82.8 against 87.7 (2 bits); rank 2 50.9
+// against 55.4; rank 4 equal (128.1 against 127.5).
+template <int K, int SBITS = 0>
 __global__ __launch_bounds__(64 * K * wide_tiles<K>()) __attribute__((amdgpu_waves_per_eu(3)))
 void k_matvec_wide(const FusedIo io, const DeviceTables *__restrict__ tab)
 {
@@ -597,12 +572,15 @@ void k_matvec_wide(const FusedIo io, const DeviceTables *__restrict__ tab)
     s_m2[1][i >> 7][(i >> 6) & 1][i & 63] =
         *reinterpret_cast<const uint64_t *>(tab->m16_inv2[i >> 7][(i >> 6) & 1][i & 63]);
   }
+  constexpr int kPk = SBITS ? SBITS - 2 : 0;
   for (int i = threadIdx.x; i < 2 * 64; i += 64 * W) {
-    s_m1[0][i >> 6][i & 63] = *reinterpret_cast<const uint64_t *>(tab->m16_fwd1[0][i >> 6][i & 63]);
+    s_m1[0][i >> 6][i & 63] =
+        *reinterpret_cast<const uint64_t *>(SBITS ? tab->m16_fwd1pk[kPk][i >> 6][i & 63]
+                                                  : tab->m16_fwd1[0][i >> 6][i & 63]);
     s_m1[1][i >> 6][i & 63] = *reinterpret_cast<const uint64_t *>(tab->m16_inv1p[0][i >> 6][i & 63]);
   }
   for (int i = threadIdx.x; i < 16; i += 64 * W) {
-    s_b1[0][i] = tab->bias_fwd1[0][i];
+    s_b1[0][i] = SBITS ? tab->bias_fwd1pk[0][i] : tab->bias_fwd1[0][i];
     s_b1[1][i] = tab->bias_inv1p[0][i];
   }
   __syncthreads();
@@ -637,14 +615,36 @@ void k_matvec_wide(const FusedIo io, const DeviceTables *__restrict__ tab)
                            (int)role_stride<kFusedMatvec, K>(role), lane);
   };
   // s_w of this sub-tile's first tile; later ones are prefetched in phase 3.
-  v4i sx[P / 2];
-  {
-    const size_t t0 = tile_of(0);
-    const auto xr = rt(rX, t0, nvalid_of(t0));
+  // Packed: lane (g, t) takes the bytes of rows 4g..4g+3 of poly wi of
+  // instance t, through a descriptor over the tile's valid instances.
+  constexpr int PPB = SBITS ? packed_poly_bytes<SBITS>() : 1;
+  constexpr int NDW = SBITS ? packed_lane_dwords<SBITS>() : 4;
+  v4i sx[SBITS ? 1 : P / 2];
+  uint32_t pv[NDW];
+  auto load_x = [&](size_t t) {
+    const int nv = nvalid_of(t);
+    if constexpr (SBITS == 0) {
+      const auto xr = rt(rX, t, nv);
 #pragma unroll
-    for (int it = 0; it < P / 2; it++)
-      sx[it] = xr.load(it, wi);
-  }
+      for (int it = 0; it < P / 2; it++)
+        sx[it] = xr.load(it, wi);
+    } else {
+      const __amdgpu_buffer_rsrc_t xr =
+          span_rsrc(nv ? static_cast<const uint8_t *>(io.x) + t * P * K * PPB : nullptr,
+                    (uint32_t)nv * K * PPB);
+#pragma unroll
+      for (int i = 0; i < NDW / 4; i++) {
+        const v4i x = __builtin_bit_cast(
+            v4i, __builtin_amdgcn_raw_buffer_load_b128(xr, tt * K * PPB + g * NDW * 4,
+                                                       wi * PPB + i * 16, kBufferKeep));
+        pv[4 * i] = (uint32_t)x[0];
+        pv[4 * i + 1] = (uint32_t)x[1];
+        pv[4 * i + 2] = (uint32_t)x[2];
+        pv[4 * i + 3] = (uint32_t)x[3];
+      }
+    }
+  };
+  load_x(tile_of(0));
   vm_drain();  // the loop's waits for s_w then count the previous tile's stores
 
   for (size_t base = 0; base < ntiles; base += per_iter) {
@@ -654,10 +654,12 @@ void k_matvec_wide(const FusedIo io, const DeviceTables *__restrict__ tab)
     const auto mr = rt(rMat, tile, nvalid);
     v4i a[kWidePre][K];
     wave_sync();
+    if constexpr (SBITS == 0) {
 #pragma unroll
-    for (int it = 0; it < P / 2; it++) {
-      const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
-      *reinterpret_cast<v4i *>(slab + pp * kStride + ch * 8) = sx[it];
+      for (int it = 0; it < P / 2; it++) {
+        const int q = it * 64 + lane, pp = q >> 5, ch = q & 31;
+        *reinterpret_cast<v4i *>(slab + pp * kStride + ch * 8) = sx[it];
+      }
     }
 #pragma unroll
     for (int c = 0; c < kWidePre; c++)
@@ -665,8 +667,12 @@ void k_matvec_wide(const FusedIo io, const DeviceTables *__restrict__ tab)
       for (int j = 0; j < K; j++)
         a[c][j] = mr.load(c, wi * K + j);
     if (nvalid) {
-      wave_sync();
-      pass_m16<kFormInter>(pslab, g, m1(0, 0), m1(0, 1), b1(0));
+      if constexpr (SBITS == 0) {
+        wave_sync();
+        pass_m16<kFormInter>(pslab, g, m1(0, 0), m1(0, 1), b1(0));
+      } else {
+        pass_m16_packed<SBITS>(pslab, g, pv, m1(0, 0), m1(0, 1), b1(0));
+      }
       wave_sync();
       pass_c16<kFormInter>(pslab, g, f2lo, f2hi, s_bf2);
     }
@@ -710,13 +716,7 @@ void k_matvec_wide(const FusedIo io, const DeviceTables *__restrict__ tab)
       *reinterpret_cast<v4i *>(slab + pp * kStride + ch * 8) =
           v4i{(int)res[it][0], (int)res[it][1], (int)res[it][2], (int)res[it][3]};
     }
-    {
-      const size_t nt = tile + per_iter;
-      const auto xr = rt(rX, nt, nvalid_of(nt));
-#pragma unroll
-      for (int it = 0; it < P / 2; it++)
-        sx[it] = xr.load(it, wi);
-    }
+    load_x(tile + per_iter);
     if (nvalid) {
       wave_sync();
       pass_c16<kFormInter>(pslab, g, i2lo, i2hi, s_bi2);
@@ -1012,22 +1012,19 @@ int launch_fused(int k, int mode, int s_bits, const FusedIo &io, const DeviceTab
     const dim3 grid((unsigned)(ntiles < cap ? ntiles : cap));
     return sink_launch(stream, kernel, grid, dim3(64 * waves), io, tab);
   };
-  auto with_k = [&](auto mode_c, auto bits_c, auto table_c) {
-    constexpr int M = decltype(mode_c)::value, B = decltype(bits_c)::value;
+  auto with_k = [&](auto mode_c, auto table_c) {
+    constexpr int M = decltype(mode_c)::value;
     constexpr bool T = decltype(table_c)::value;
     switch (k) {
-      case 2: return go(k_fused<2, M, B, T>, fused_waves<M, 2>());
-      case 3: return go(k_fused<3, M, B, T>, fused_waves<M, 3>());
-      case 4: return go(k_fused<4, M, B, T>, fused_waves<M, 4>());
+      case 2: return go(k_fused<2, M, T>, fused_waves<M, 2>());
+      case 3: return go(k_fused<3, M, T>, fused_waves<M, 3>());
+      case 4: return go(k_fused<4, M, T>, fused_waves<M, 4>());
       default: return -1;
     }
   };
   using std::integral_constant;
-  using Lin = std::false_type;
-  using Tab = std::true_type;
   auto by_table = [&](auto mode_c) {
-    return io.inst ? with_k(mode_c, integral_constant<int, 0>{}, Tab{})
-                   : with_k(mode_c, integral_constant<int, 0>{}, Lin{});
+    return io.inst ? with_k(mode_c, std::true_type{}) : with_k(mode_c, std::false_type{});
   };
   if (io.inst && s_bits)
     return -1;
@@ -1038,21 +1035,27 @@ int launch_fused(int k, int mode, int s_bits, const FusedIo &io, const DeviceTab
   };
   switch (mode) {
     case kFusedMatvec: {
-      const integral_constant<int, kFusedMatvec> mc{};
-      if (s_bits == 0 && !io.inst)
-        switch (k) {
-          case 2: return wide(k_matvec_wide<2>, 2 * wide_tiles<2>());
-          case 3: return wide(k_matvec_wide<3>, 3 * wide_tiles<3>());
-          case 4: return wide(k_matvec_wide<4>, 4 * wide_tiles<4>());
-          default: break;
+      // Contiguous batches (int16 or packed s): the wide kernel; plans: the
+      // table-driven fused kernel.
+      if (!io.inst) {
+        auto wide_k = [&](auto bits_c) {
+          constexpr int B = decltype(bits_c)::value;
+          switch (k) {
+            case 2: return wide(k_matvec_wide<2, B>, 2 * wide_tiles<2>());
+            case 3: return wide(k_matvec_wide<3, B>, 3 * wide_tiles<3>());
+            case 4: return wide(k_matvec_wide<4, B>, 4 * wide_tiles<4>());
+            default: return -1;
+          }
+        };
+        switch (s_bits) {
+          case 0: return wide_k(integral_constant<int, 0>{});
+          case 2: return wide_k(integral_constant<int, 2>{});
+          case 3: return wide_k(integral_constant<int, 3>{});
+          case 4: return wide_k(integral_constant<int, 4>{});
+          default: return -1;
         }
-      switch (s_bits) {
-        case 0: return by_table(mc);
-        case 2: return with_k(mc, integral_constant<int, 2>{}, Lin{});
-        case 3: return with_k(mc, integral_constant<int, 3>{}, Lin{});
-        case 4: return with_k(mc, integral_constant<int, 4>{}, Lin{});
-        default: return -1;
       }
+      return with_k(integral_constant<int, kFusedMatvec>{}, std::true_type{});
     }
     case kFusedKeygen:
       return s_bits ? -1 : by_table(integral_constant<int, kFusedKeygen>{});

@@ -91,7 +91,7 @@ def test_no_spills_and_tile_kernel_occupancy(compiled):
     assert seen > 50, "kernel remarks not found"
     assert four >= 30, "four-wave kernels not found"
     assert queued == 6, "queued packed forwards: 2 output forms x 3 widths"
-    assert wide_mv == 3, "wide matvec kernels for ranks 2, 3 and 4"
+    assert wide_mv == 12, "wide matvec kernels for ranks 2, 3, 4 x s int16 or packed in 2, 3, 4 bits"
 
 
 _WIDE_STORE = re.compile(r"^(buffer|global|flat)_store_dwordx(3|4)\b.*?v\[(\d+):(\d+)\]")
