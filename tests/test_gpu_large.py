@@ -153,6 +153,7 @@ def test_concurrent_host_threads(tk, gpu, oracle):
 
 
 @pytest.mark.parametrize("mode,k,n", [("matvec", 3, 120000), ("matvec_packed3", 3, 120000),
+                                      ("matvec_packed2", 2, 120000), ("matvec_packed4", 4, 120000),
                                       ("keygen", 2, 120000), ("keygen", 3, 120000),
                                       ("keygen", 4, 120000), ("encrypt", 2, 120000),
                                       ("encrypt", 3, 100000), ("encrypt", 4, 120000),
@@ -181,11 +182,13 @@ def test_fused_batches_larger_than_the_grid(tk, gpu, oracle, mode, k, n):
 
     print(f"fused batch larger than the grid: {mode}, k = {k}, {n} instances", flush=True)
     if mode.startswith("matvec"):
-        A, s = rnd(n, k, k, N), rnd(n, k, N, lo=-2, hi=3)
+        # packed s must fit its width: [-2, 1] at 2 bits, CBD [-2, 2] otherwise
+        A, s = rnd(n, k, k, N), rnd(n, k, N, lo=-2, hi=2 if mode.endswith("2") else 3)
         if mode == "matvec":
             got = pick(tk.matvec(A, s, k), k, N)
         else:
-            got = pick(tk.matvec_packed(A, tk.pack_small(s, 3), k, 3), k, N)
+            bits = int(mode[-1])
+            got = pick(tk.matvec_packed(A, tk.pack_small(s, bits), k, bits), k, N)
         want = oracle.matvec(pick(A, k, k, N), pick(s, k, N), k).reshape(got.shape)
         assert np.array_equal(got, centred(want))  # the oracle returns invntt's (-q, q) values
     elif mode == "keygen":
